@@ -1,0 +1,141 @@
+"""ctypes bindings to the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg use this module as the checker and as the reported CPU
+baseline. The product path never imports it.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+
+
+def build(arch: str | None = None, out: str | None = None) -> str:
+    """Compile the oracle with its Makefile; returns the library path."""
+    cmd = ["make", "-s", "-C", _HERE]
+    if arch:
+        cmd.append(f"ARCH={arch}")
+    if out:
+        cmd += [f"OUT={out}", "-B"]
+    subprocess.run(cmd, check=True)
+    return out or os.path.join(_HERE, "liboracle.so")
+
+
+def _bind(lib):
+    lib.oracle_t_abc.restype = ctypes.c_double
+    lib.oracle_t_abc.argtypes = [ctypes.c_int64, ctypes.c_int64, _dp, _dp, _dp, _dp, _dp, _dp,
+                                 _i64p, ctypes.c_int64, ctypes.c_int, _dp]
+    lib.oracle_t_ijk.restype = ctypes.c_double
+    lib.oracle_t_ijk.argtypes = [ctypes.c_int64, ctypes.c_int64, _dp, _dp, _dp, _dp, _dp, _dp,
+                                 _i64p, ctypes.c_int64, ctypes.c_int, _dp]
+    lib.oracle_t_brute.restype = ctypes.c_double
+    lib.oracle_t_brute.argtypes = [ctypes.c_int64, ctypes.c_int64, _dp, _dp, _dp, _dp, _dp, _dp]
+    lib.oracle_t_grad.restype = None
+    lib.oracle_t_grad.argtypes = [ctypes.c_int64, ctypes.c_int64, _dp, _dp, _dp, _dp, _dp, _dp,
+                                  _dp, _dp, _dp, _dp, _dp]
+    lib.oracle_num_unique_triples.restype = ctypes.c_int64
+    lib.oracle_num_unique_triples.argtypes = [ctypes.c_int64]
+    lib.oracle_num_unique_ijk.restype = ctypes.c_int64
+    lib.oracle_num_unique_ijk.argtypes = [ctypes.c_int64]
+    lib.oracle_unique_ijk.restype = None
+    lib.oracle_unique_ijk.argtypes = [ctypes.c_int64, _i32p]
+    return lib
+
+
+def load(path: str | None = None):
+    """Load (building first if needed) the oracle library."""
+    global _LIB
+    if path is not None:
+        return _bind(ctypes.CDLL(path))
+    if _LIB is None:
+        so = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(so):
+            build()
+        _LIB = _bind(ctypes.CDLL(so))
+    return _LIB
+
+
+def _f(a: np.ndarray) -> np.ndarray:
+    a = np.asarray(a, dtype=np.float64)
+    if a.ndim > 1 and not a.flags.f_contiguous:
+        a = np.asfortranarray(a)
+    return a
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+def _args(inp):
+    arrs = [_f(inp.e), _f(inp.t1), _f(inp.t2), _f(inp.ovvv), _f(inp.ovoo), _f(inp.ovov)]
+    return arrs
+
+
+def num_threads() -> int:
+    return max(1, len(os.sched_getaffinity(0)))
+
+
+def t_abc(inp, items=None, nthreads=None, item_energy=False, lib=None):
+    """E(T) by the abc-driven restatement of RCCSD_TriplesCorrection.
+
+    items: optional int64 indices of unique triples a>=b>=c to evaluate (then the
+    return value is 2 * the sum over those). item_energy=True also returns the
+    per-triple energies."""
+    lib = lib or load()
+    arrs = _args(inp)
+    it = None if items is None else np.ascontiguousarray(items, dtype=np.int64)
+    n = lib.oracle_num_unique_triples(inp.v) if it is None else len(it)
+    out = np.zeros(max(n, 1)) if item_energy else None
+    E = lib.oracle_t_abc(inp.o, inp.v, *[_p(a) for a in arrs],
+                         it.ctypes.data_as(_i64p) if it is not None else None,
+                         0 if it is None else len(it), nthreads or num_threads(), _p(out))
+    return (E, out[:n]) if item_energy else E
+
+
+def t_ijk(inp, items=None, nthreads=None, lib=None):
+    """(E, F) by the ijk-driven form: F[t] = sum_abc W r3(Y)/D for the listed unique
+    occupied triples (all when items is None), E = sum 2/deg F."""
+    lib = lib or load()
+    arrs = _args(inp)
+    it = None if items is None else np.ascontiguousarray(items, dtype=np.int64)
+    n = lib.oracle_num_unique_ijk(inp.o) if it is None else len(it)
+    F = np.zeros(max(n, 1))
+    E = lib.oracle_t_ijk(inp.o, inp.v, *[_p(a) for a in arrs],
+                         it.ctypes.data_as(_i64p) if it is not None else None,
+                         0 if it is None else len(it), nthreads or num_threads(), _p(F))
+    return E, F[:n]
+
+
+def t_brute(inp, lib=None):
+    lib = lib or load()
+    arrs = _args(inp)
+    return lib.oracle_t_brute(inp.o, inp.v, *[_p(a) for a in arrs])
+
+
+def t_grad(inp, lib=None):
+    """dict of FP64 gradients of E(T) in the input layouts."""
+    lib = lib or load()
+    arrs = _args(inp)
+    g = {k: np.zeros(getattr(inp, k).shape, order="F") for k in ("ovvv", "ovoo", "ovov", "t1", "t2")}
+    lib.oracle_t_grad(inp.o, inp.v, *[_p(a) for a in arrs],
+                      _p(g["ovvv"]), _p(g["ovoo"]), _p(g["ovov"]), _p(g["t1"]), _p(g["t2"]))
+    return g
+
+
+def unique_ijk(o: int, lib=None) -> np.ndarray:
+    lib = lib or load()
+    n = lib.oracle_num_unique_ijk(o)
+    out = np.zeros((max(n, 1), 3), dtype=np.int32)
+    lib.oracle_unique_ijk(o, out.ctypes.data_as(_i32p))
+    return out[:n]
