@@ -1,0 +1,121 @@
+/*
+ * scess_gpu.h - C ABI of libscess_gpu.so, the MI355X (gfx950) closed-shell
+ * CCSD(T) hot path of SCESS.
+ *
+ * The reference has no C ABI: its boundary is the C++ method API of
+ * libQuantumSCESS, src/Quantum/CC/CC.h:136-164. The entry points below are what
+ * a binding of that API to this library calls; the C++ mirror of CC.h in
+ * scess_amd/csrc/cc_api.{h,cpp} and the Python mirror in scess_amd/triples.py
+ * sit on top of them.
+ *
+ * Conventions
+ *   - All host arrays are FP64 in Eigen's default ColMajor layout (first index
+ *     fastest), the layout of the reference's Eigen::Tensor<double, N>.
+ *   - Integral blocks are in chemists' notation, named as in
+ *     src/Quantum/CC/RCCSD.cpp:31-44:  ovvv(i,a,b,f) = (ia|bf),
+ *     ovoo(i,a,m,j) = (ia|mj), ovov(i,a,j,b) = (ia|jb).
+ *   - Return codes: SCESS_OK (0) or a negative SCESS_ERR_*; no exception
+ *     crosses the ABI. scess_last_error() describes the last failure.
+ *   - One context per host thread. Calls are synchronous (they return after the
+ *     device work has finished) and run on the context's own HIP stream.
+ *   - Cancellation: an optional should_stop callback is polled between batches
+ *     of occupied triples (the reference polls DH::Thread::ShouldStop() per
+ *     triple, src/Quantum/CC/RCCSD_T.cpp:123,161-162); a cancelled call
+ *     returns SCESS_CANCELLED and writes 0.0 as the energy, as the reference
+ *     returns 0.0.
+ */
+#ifndef SCESS_GPU_H
+#define SCESS_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct scess_ctx scess_ctx;
+
+enum {
+    SCESS_OK = 0,
+    SCESS_ERR_ARG = -1,    /* invalid argument or unsupported size */
+    SCESS_ERR_HIP = -2,    /* a HIP runtime call failed */
+    SCESS_ERR_OOM = -3,    /* device memory exhausted */
+    SCESS_ERR_STATE = -4,  /* call out of order (e.g. forward before inputs) */
+    SCESS_CANCELLED = -5   /* should_stop returned non-zero */
+};
+
+/* Precision of the W contractions; denominators, V, r3 and every energy
+ * reduction are FP64 in all modes. */
+enum {
+    SCESS_PREC_FP64 = 0,   /* FP64 operands, FP64 MFMA */
+    SCESS_PREC_FP32 = 1,   /* FP32 operands, FP32 MFMA, FP32 accumulation */
+    SCESS_PREC_BF16X3 = 2  /* x = hi + lo in bf16; hi*hi + hi*lo + lo*hi on bf16 MFMA, FP32 accumulation */
+};
+
+/* Inputs of the (T) functional. Replaces the (eri, orbitalEnergies, nOcc, t1,
+ * t2) arguments of RCCSD_TriplesCorrection (src/Quantum/CC/CC.h:154-158) by
+ * the three integral blocks the functional reads (RCCSD_T.cpp:27-56). */
+typedef struct scess_t_in {
+    int64_t o;            /* occupied orbitals (nOcc) */
+    int64_t v;            /* virtual orbitals */
+    const double *e;      /* [o+v] orbital energies, occupied first */
+    const double *t1;     /* (i,a)      o x v */
+    const double *t2;     /* (i,j,a,b)  o x o x v x v, t_ij^ab = t_ji^ba */
+    const double *ovvv;   /* (i,a,b,f)  (ia|bf) */
+    const double *ovoo;   /* (i,a,m,j)  (ia|mj) */
+    const double *ovov;   /* (i,a,j,b)  (ia|jb) */
+} scess_t_in;
+
+typedef struct scess_t_stats {
+    double seconds;            /* device time of the whole call (HIP events) */
+    double w_build_ms;         /* summed device time of the W-build launches */
+    double epilogue_ms;        /* summed device time of the epilogue launches */
+    int64_t w_build_launches;
+    int64_t epilogue_launches;
+    int64_t n_ijk;             /* unique occupied triples i>=j>=k evaluated */
+    double flops_algorithmic;  /* 2 o^3 v^3 (o+v) */
+    double flops_executed;     /* MFMA work actually issued, padding included */
+} scess_t_stats;
+
+/* Gradients of E(T), same layouts and sizes as the inputs (host, FP64). */
+typedef struct scess_t_grad {
+    double *ovvv;
+    double *ovoo;
+    double *ovov;
+    double *t1;
+    double *t2;
+} scess_t_grad;
+
+/* -- context ---------------------------------------------------------------- */
+int scess_create(scess_ctx **ctx, int device);
+void scess_destroy(scess_ctx *ctx);
+const char *scess_last_error(const scess_ctx *ctx);
+int scess_set_should_stop(scess_ctx *ctx, int (*should_stop)(void *), void *user);
+/* Occupied-triple batch size (0 = automatic). */
+int scess_set_batch(scess_ctx *ctx, int64_t nbatch);
+
+/* -- (T) ----------------------------------------------------------------------
+ * scess_t_set_inputs uploads the inputs once and repacks them on the device
+ * into the layouts of every precision in prec_mask (bit 1 << SCESS_PREC_*);
+ * they stay resident in HBM for any number of forward/backward calls. This is
+ * the device counterpart of the serial repack of RCCSD_T.cpp:27-56. */
+int scess_t_set_inputs(scess_ctx *ctx, const scess_t_in *in, int prec_mask);
+
+/* E(T) (RCCSD_TriplesCorrection, src/Quantum/CC/RCCSD_T.cpp:17-168) from the
+ * resident inputs. stats may be NULL. */
+int scess_t_forward(scess_ctx *ctx, int prec, double *e_t, scess_t_stats *stats);
+
+/* As scess_t_forward, and optionally also:
+ *   F_ijk     [scess_t_num_ijk] per unique occupied triple (enumeration i outer,
+ *             j <= i, k <= j), F = sum_abc W r3(Y)/D before the 2/deg weight;
+ *   partials  [scess_t_num_ijk * scess_t_num_orbits] per (ijk, virtual block
+ *             orbit) pieces of F, for local finite differences. */
+int scess_t_forward_ex(scess_ctx *ctx, int prec, double *e_t, double *F_ijk, double *partials,
+                       scess_t_stats *stats);
+int64_t scess_t_num_ijk(const scess_ctx *ctx);
+int64_t scess_t_num_orbits(const scess_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SCESS_GPU_H */

@@ -1,0 +1,492 @@
+// C ABI of libscess_gpu.so (include/scess_gpu.h): context, resident inputs and
+// the (T) driver that schedules the gfx950 kernels of triples_kernels.hip.
+#include "../../include/scess_gpu.h"
+#include "triples_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace scess;
+
+struct TriplesState {
+    bool have = false;
+    int o = 0, v = 0, vR = 0;
+    bool have_prec[3] = {false, false, false};
+    Operands ops[3];
+    double *S = nullptr, *t1 = nullptr, *e = nullptr;  // FP64 epilogue inputs
+    std::vector<int> ijk_all;                          // 3 per unique occupied triple
+    std::vector<int> work;                             // indices (into ijk_all) with i != k
+    int *d_ijk_all = nullptr;
+    std::vector<int> orbits;
+    int *d_orbits = nullptr;
+    int nb = 0, norb = 0;
+    // work buffers
+    void *W1 = nullptr, *W2 = nullptr;
+    size_t wbytes = 0;
+    int *d_work_ijk = nullptr, *d_work_index = nullptr;  // per work item (i != k)
+    double *partials = nullptr;
+    size_t partials_cap = 0;
+    double *F = nullptr, *E = nullptr;
+};
+
+struct scess_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int (*should_stop)(void *) = nullptr;
+    void *user = nullptr;
+    int64_t nbatch_req = 0;
+    TriplesState t;
+};
+
+namespace {
+
+int fail(scess_ctx *c, int code, const std::string &msg) {
+    if (c)
+        c->err = msg;
+    return code;
+}
+
+int hip_fail(scess_ctx *c, hipError_t e, const char *what) {
+    const int code = (e == hipErrorOutOfMemory) ? SCESS_ERR_OOM : SCESS_ERR_HIP;
+    return fail(c, code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(c, expr)                                  \
+    do {                                                  \
+        hipError_t e__ = (expr);                          \
+        if (e__ != hipSuccess)                            \
+            return hip_fail((c), e__, #expr);             \
+    } while (0)
+
+template <class T>
+void dfree(T *&p) {
+    if (p) {
+        (void)hipFree((void *)p);
+        p = nullptr;
+    }
+}
+
+void free_triples(TriplesState &t) {
+    for (auto &op : t.ops) {
+        dfree(op.X);
+        dfree(op.T);
+        dfree(op.Rn);
+        dfree(op.TM);
+        op = Operands{};
+    }
+    dfree(t.S);
+    dfree(t.t1);
+    dfree(t.e);
+    dfree(t.d_ijk_all);
+    dfree(t.d_orbits);
+    dfree(t.W1);
+    dfree(t.W2);
+    dfree(t.d_work_ijk);
+    dfree(t.d_work_index);
+    dfree(t.partials);
+    dfree(t.F);
+    dfree(t.E);
+    t = TriplesState{};
+}
+
+int upload(scess_ctx *c, const double *h, size_t n, double **d) {
+    HIP_TRY(c, hipMalloc((void **)d, sizeof(double) * std::max<size_t>(n, 1)));
+    HIP_TRY(c, hipMemcpyAsync(*d, h, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    return SCESS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int scess_create(scess_ctx **out, int device) {
+    if (!out)
+        return SCESS_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+        return SCESS_ERR_HIP;
+    if (device < 0 || device >= n)
+        return SCESS_ERR_ARG;
+    auto *c = new scess_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+        delete c;
+        return SCESS_ERR_HIP;
+    }
+    *out = c;
+    return SCESS_OK;
+}
+
+void scess_destroy(scess_ctx *c) {
+    if (!c)
+        return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    free_triples(c->t);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char *scess_last_error(const scess_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int scess_set_should_stop(scess_ctx *c, int (*fn)(void *), void *user) {
+    if (!c)
+        return SCESS_ERR_ARG;
+    c->should_stop = fn;
+    c->user = user;
+    return SCESS_OK;
+}
+
+int scess_set_batch(scess_ctx *c, int64_t nbatch) {
+    if (!c || nbatch < 0)
+        return SCESS_ERR_ARG;
+    c->nbatch_req = nbatch;
+    return SCESS_OK;
+}
+
+int64_t scess_t_num_ijk(const scess_ctx *c) { return c && c->t.have ? (int64_t)c->t.ijk_all.size() / 3 : 0; }
+
+int64_t scess_t_num_orbits(const scess_ctx *c) { return c && c->t.have ? c->t.norb : 0; }
+
+int scess_t_set_inputs(scess_ctx *c, const scess_t_in *in, int prec_mask) {
+    if (!c)
+        return SCESS_ERR_ARG;
+    if (!in || in->o < 1 || in->v < 1 || !in->e || !in->t1 || !in->t2 || !in->ovvv || !in->ovoo || !in->ovov)
+        return fail(c, SCESS_ERR_ARG, "scess_t_set_inputs: null array or non-positive size");
+    if (in->o > 4096 || in->v > 4096)
+        return fail(c, SCESS_ERR_ARG, "scess_t_set_inputs: o and v must be <= 4096");
+    if ((prec_mask & 7) == 0 || (prec_mask & ~7))
+        return fail(c, SCESS_ERR_ARG, "scess_t_set_inputs: prec_mask must be a non-empty subset of bits 0..2");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    free_triples(c->t);
+    TriplesState &t = c->t;
+    const int64_t o = in->o, v = in->v;
+    t.o = (int)o;
+    t.v = (int)v;
+    t.vR = (int)round_up(v, 16);
+
+    // raw ColMajor inputs (staging)
+    double *r_ovvv = nullptr, *r_ovoo = nullptr, *r_ovov = nullptr, *r_t2 = nullptr;
+    int rc = SCESS_OK;
+    auto cleanup = [&]() {
+        dfree(r_ovvv);
+        dfree(r_ovoo);
+        dfree(r_ovov);
+        dfree(r_t2);
+    };
+    if ((rc = upload(c, in->ovvv, (size_t)(o * v * v * v), &r_ovvv)) ||
+        (rc = upload(c, in->ovoo, (size_t)(o * v * o * o), &r_ovoo)) ||
+        (rc = upload(c, in->ovov, (size_t)(o * v * o * v), &r_ovov)) ||
+        (rc = upload(c, in->t2, (size_t)(o * o * v * v), &r_t2))) {
+        cleanup();
+        free_triples(t);
+        return rc;
+    }
+    // t1 as [o][v] (ColMajor (i,a) -> row-major i)
+    std::vector<double> t1r((size_t)(o * v));
+    for (int64_t i = 0; i < o; i++)
+        for (int64_t a = 0; a < v; a++)
+            t1r[(size_t)(i * v + a)] = in->t1[i + o * a];
+    if ((rc = upload(c, t1r.data(), t1r.size(), &t.t1)) || (rc = upload(c, in->e, (size_t)(o + v), &t.e))) {
+        cleanup();
+        free_triples(t);
+        return rc;
+    }
+    auto pack = [&](int mode, int prec, const double *src, void **dst, int64_t rows, int64_t kpad) -> int {
+        const size_t bytes = (size_t)rows * (size_t)kpad * (prec == P64 ? 8 : 4);
+        hipError_t e = hipMalloc(dst, std::max<size_t>(bytes, 16));
+        if (e != hipSuccess)
+            return hip_fail(c, e, "hipMalloc (packed operand)");
+        e = launch_pack(mode, prec, src, *dst, (int)o, (int)v, kpad, c->stream);
+        if (e != hipSuccess)
+            return hip_fail(c, e, "pack kernel");
+        return SCESS_OK;
+    };
+    if ((rc = pack(PACK_S, P64, r_ovov, (void **)&t.S, o * o * v, v))) {
+        cleanup();
+        free_triples(t);
+        return rc;
+    }
+    for (int p = 0; p < 3; p++) {
+        if (!(prec_mask & (1 << p)))
+            continue;
+        Operands &op = t.ops[p];
+        op.vK = round_up(v, kChunk[p]);
+        op.oK = round_up(o, kChunk[p]);
+        op.rbv = op.vK * kElemBytes[p];
+        op.rbo = op.oK * kElemBytes[p];
+        if ((rc = pack(PACK_X, p, r_ovvv, &op.X, o * v * v, op.vK)) ||
+            (rc = pack(PACK_T, p, r_t2, &op.T, o * o * v, op.vK)) ||
+            (rc = pack(PACK_RN, p, r_ovoo, &op.Rn, o * o * v, op.oK)) ||
+            (rc = pack(PACK_TM, p, r_t2, &op.TM, o * v * v, op.oK))) {
+            cleanup();
+            free_triples(t);
+            return rc;
+        }
+        t.have_prec[p] = true;
+    }
+    hipError_t se = hipStreamSynchronize(c->stream);
+    cleanup();
+    if (se != hipSuccess) {
+        free_triples(t);
+        return hip_fail(c, se, "scess_t_set_inputs");
+    }
+
+    // unique occupied triples i >= j >= k (RCCSD_T.cpp enumerates a >= b >= c the same way)
+    for (int i = 0; i < o; i++)
+        for (int j = 0; j <= i; j++)
+            for (int k = 0; k <= j; k++) {
+                if (i != k)
+                    t.work.push_back((int)(t.ijk_all.size() / 3));
+                t.ijk_all.push_back(i);
+                t.ijk_all.push_back(j);
+                t.ijk_all.push_back(k);
+            }
+    // virtual block orbits A >= B >= C over 8-wide blocks
+    t.nb = t.vR / 8;
+    for (int A = 0; A < t.nb; A++)
+        for (int B = 0; B <= A; B++)
+            for (int C = 0; C <= B; C++) {
+                t.orbits.push_back(A);
+                t.orbits.push_back(B);
+                t.orbits.push_back(C);
+            }
+    t.norb = (int)(t.orbits.size() / 3);
+    HIP_TRY(c, hipMalloc((void **)&t.d_ijk_all, sizeof(int) * t.ijk_all.size()));
+    HIP_TRY(c, hipMemcpy(t.d_ijk_all, t.ijk_all.data(), sizeof(int) * t.ijk_all.size(), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMalloc((void **)&t.d_orbits, sizeof(int) * t.orbits.size()));
+    HIP_TRY(c, hipMemcpy(t.d_orbits, t.orbits.data(), sizeof(int) * t.orbits.size(), hipMemcpyHostToDevice));
+    {
+        std::vector<int> wijk;
+        for (int w : t.work) {
+            wijk.push_back(t.ijk_all[3 * w]);
+            wijk.push_back(t.ijk_all[3 * w + 1]);
+            wijk.push_back(t.ijk_all[3 * w + 2]);
+        }
+        const size_t nw = std::max<size_t>(t.work.size(), 1);
+        HIP_TRY(c, hipMalloc((void **)&t.d_work_ijk, sizeof(int) * 3 * nw));
+        HIP_TRY(c, hipMalloc((void **)&t.d_work_index, sizeof(int) * nw));
+        if (!t.work.empty()) {
+            HIP_TRY(c, hipMemcpy(t.d_work_ijk, wijk.data(), sizeof(int) * wijk.size(), hipMemcpyHostToDevice));
+            HIP_TRY(c, hipMemcpy(t.d_work_index, t.work.data(), sizeof(int) * t.work.size(), hipMemcpyHostToDevice));
+        }
+    }
+    HIP_TRY(c, hipMalloc((void **)&t.F, sizeof(double) * (t.ijk_all.size() / 3)));
+    HIP_TRY(c, hipMalloc((void **)&t.E, sizeof(double)));
+    t.have = true;
+    return SCESS_OK;
+}
+
+int scess_t_forward(scess_ctx *c, int prec, double *e_t, scess_t_stats *stats) {
+    return scess_t_forward_ex(c, prec, e_t, nullptr, nullptr, stats);
+}
+
+int scess_t_forward_ex(scess_ctx *c, int prec, double *e_t, double *F_ijk, double *partials_out,
+                       scess_t_stats *stats) {
+    if (!c)
+        return SCESS_ERR_ARG;
+    if (!e_t)
+        return fail(c, SCESS_ERR_ARG, "scess_t_forward: e_t is null");
+    *e_t = 0.0;
+    if (prec < 0 || prec > 2)
+        return fail(c, SCESS_ERR_ARG, "scess_t_forward: unknown precision");
+    TriplesState &t = c->t;
+    if (!t.have)
+        return fail(c, SCESS_ERR_STATE, "scess_t_forward: no inputs (call scess_t_set_inputs first)");
+    if (!t.have_prec[prec])
+        return fail(c, SCESS_ERR_STATE, "scess_t_forward: precision not packed by scess_t_set_inputs");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const int64_t o = t.o, v = t.v, vR = t.vR;
+    const int64_t nall = (int64_t)t.ijk_all.size() / 3;
+    const int64_t nwork = (int64_t)t.work.size();
+    const size_t accb = prec == P64 ? 8 : 4;
+    const int64_t wstride = v * vR * vR;
+
+    // batch size: bounded W footprint (2 buffers), at least 1
+    int64_t nbatch = c->nbatch_req;
+    if (nbatch <= 0) {
+        const double per = 2.0 * (double)wstride * (double)accb;
+        nbatch = (int64_t)std::max(1.0, std::min(16.0, 4.0e9 / per));
+    }
+    nbatch = std::max<int64_t>(1, std::min<int64_t>(nbatch, std::max<int64_t>(nwork, 1)));
+    if (nbatch > 65535)
+        nbatch = 65535;
+    const size_t wneed = (size_t)nbatch * (size_t)wstride * accb;
+    if (t.wbytes < wneed) {
+        dfree(t.W1);
+        dfree(t.W2);
+        t.wbytes = 0;
+        HIP_TRY(c, hipMalloc(&t.W1, wneed));
+        HIP_TRY(c, hipMalloc(&t.W2, wneed));
+        t.wbytes = wneed;
+    }
+    // per-orbit partials: whole job if the caller wants them, else one batch
+    const size_t pneed = (size_t)(partials_out ? std::max<int64_t>(nwork, 1) : nbatch) * (size_t)t.norb;
+    if (t.partials_cap < pneed) {
+        dfree(t.partials);
+        HIP_TRY(c, hipMalloc((void **)&t.partials, sizeof(double) * pneed));
+        t.partials_cap = pneed;
+    }
+    HIP_TRY(c, hipMemsetAsync(t.F, 0, sizeof(double) * nall, c->stream));
+
+    const Operands &op = t.ops[prec];
+    const int blk = wbuild_tile(prec, (int)v);
+    const int BM = 2 * blk * 16;
+    WBuildArgs wa;
+    wa.X = (const char *)op.X;
+    wa.T = (const char *)op.T;
+    wa.Rn = (const char *)op.Rn;
+    wa.TM = (const char *)op.TM;
+    wa.o = (int)o;
+    wa.v = (int)v;
+    wa.vR = (int)vR;
+    wa.rbv = op.rbv;
+    wa.rbo = op.rbo;
+    wa.nchv = (int)(op.vK / kChunk[prec]);
+    wa.ncho = (int)(op.oK / kChunk[prec]);
+    wa.nt = (int)((v + BM - 1) / BM);
+    wa.ntile1 = (int)(v * wa.nt * wa.nt);
+    wa.W1 = t.W1;
+    wa.W2 = t.W2;
+    wa.wstride = wstride;
+
+    EpiArgs ea;
+    ea.W1 = t.W1;
+    ea.W2 = t.W2;
+    ea.wstride = wstride;
+    ea.o = (int)o;
+    ea.v = (int)v;
+    ea.vR = (int)vR;
+    ea.nb = t.nb;
+    ea.orbits = t.d_orbits;
+    ea.norb = t.norb;
+    ea.S = t.S;
+    ea.t1 = t.t1;
+    ea.e = t.e;
+
+    std::vector<hipEvent_t> ev;
+    auto new_event = [&](hipEvent_t *e) -> hipError_t {
+        hipError_t r = hipEventCreate(e);
+        if (r == hipSuccess)
+            ev.push_back(*e);
+        return r;
+    };
+    auto destroy_events = [&]() {
+        for (auto e : ev)
+            (void)hipEventDestroy(e);
+        ev.clear();
+    };
+    hipEvent_t e_begin, e_end;
+    if (new_event(&e_begin) != hipSuccess || new_event(&e_end) != hipSuccess) {
+        destroy_events();
+        return fail(c, SCESS_ERR_HIP, "hipEventCreate");
+    }
+    struct BatchEvents {
+        hipEvent_t a, b, c;
+    };
+    std::vector<BatchEvents> bev;
+    (void)hipEventRecord(e_begin, c->stream);
+
+    int rc = SCESS_OK;
+    bool cancelled = false;
+    for (int64_t b0 = 0; b0 < nwork; b0 += nbatch) {
+        // keep at most two batches in flight so cancellation is polled while the device works
+        if (bev.size() >= 2 && hipEventSynchronize(bev[bev.size() - 2].c) != hipSuccess) {
+            rc = fail(c, SCESS_ERR_HIP, "hipEventSynchronize");
+            break;
+        }
+        if (c->should_stop && c->should_stop(c->user)) {
+            cancelled = true;
+            break;
+        }
+        const int nb = (int)std::min<int64_t>(nbatch, nwork - b0);
+        wa.ijk = t.d_work_ijk + 3 * b0;
+        ea.ijk = t.d_work_ijk + 3 * b0;
+        hipError_t he = hipSuccess;
+        BatchEvents be{};
+        if (new_event(&be.a) != hipSuccess || new_event(&be.b) != hipSuccess || new_event(&be.c) != hipSuccess) {
+            rc = fail(c, SCESS_ERR_HIP, "hipEventCreate");
+            break;
+        }
+        double *pbase = t.partials + (partials_out ? (size_t)b0 * t.norb : 0);
+        ea.partials = pbase;
+        (void)hipEventRecord(be.a, c->stream);
+        he = launch_wbuild(prec, blk, wa, nb, c->stream);
+        if (he == hipSuccess)
+            he = hipEventRecord(be.b, c->stream);
+        if (he == hipSuccess)
+            he = launch_epilogue(prec, ea, nb, c->stream);
+        if (he == hipSuccess)
+            he = hipEventRecord(be.c, c->stream);
+        if (he == hipSuccess)
+            he = launch_reduce_partials(pbase, t.norb, t.d_work_index + b0, t.F, nb, c->stream);
+        if (he != hipSuccess) {
+            rc = hip_fail(c, he, "kernel launch");
+            break;
+        }
+        bev.push_back(be);
+    }
+    if (rc == SCESS_OK && !cancelled) {
+        hipError_t he = launch_reduce_energy(t.F, t.d_ijk_all, (int)nall, t.E, c->stream);
+        if (he != hipSuccess)
+            rc = hip_fail(c, he, "reduce_energy launch");
+    }
+    (void)hipEventRecord(e_end, c->stream);
+    hipError_t se = hipStreamSynchronize(c->stream);
+    if (rc == SCESS_OK && se != hipSuccess)
+        rc = hip_fail(c, se, "scess_t_forward");
+    if (rc == SCESS_OK && cancelled)
+        rc = fail(c, SCESS_CANCELLED, "scess_t_forward: cancelled");
+    if (rc == SCESS_OK) {
+        double E = 0.0;
+        hipError_t he = hipMemcpy(&E, t.E, sizeof(double), hipMemcpyDeviceToHost);
+        if (he == hipSuccess && F_ijk)
+            he = hipMemcpy(F_ijk, t.F, sizeof(double) * nall, hipMemcpyDeviceToHost);
+        if (he == hipSuccess && partials_out) {
+            // scatter per-work-item partials to the full unique-ijk enumeration (zeros for i == j == k)
+            std::memset(partials_out, 0, sizeof(double) * (size_t)nall * t.norb);
+            std::vector<double> hp((size_t)nwork * t.norb);
+            he = hipMemcpy(hp.data(), t.partials, sizeof(double) * hp.size(), hipMemcpyDeviceToHost);
+            for (int64_t w = 0; he == hipSuccess && w < nwork; w++)
+                std::memcpy(partials_out + (size_t)t.work[(size_t)w] * t.norb, hp.data() + (size_t)w * t.norb,
+                            sizeof(double) * t.norb);
+        }
+        if (he != hipSuccess)
+            rc = hip_fail(c, he, "result copy");
+        else
+            *e_t = E;
+    }
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, e_begin, e_end) == hipSuccess)
+            stats->seconds = ms * 1e-3;
+        for (auto &be : bev) {
+            float m1 = 0.0f, m2 = 0.0f;
+            if (hipEventElapsedTime(&m1, be.a, be.b) == hipSuccess)
+                stats->w_build_ms += m1;
+            if (hipEventElapsedTime(&m2, be.b, be.c) == hipSuccess)
+                stats->epilogue_ms += m2;
+        }
+        stats->w_build_launches = (int64_t)bev.size();
+        stats->epilogue_launches = (int64_t)bev.size();
+        stats->n_ijk = nwork;
+        stats->flops_algorithmic = 2.0 * (double)o * o * o * (double)v * v * v * (double)(o + v);
+        const double rows = (double)round_up(v, 16);
+        stats->flops_executed = (double)nwork * 2.0 * (double)v * rows * rows *
+                                (6.0 * (double)op.vK + 6.0 * (double)op.oK);
+    }
+    destroy_events();
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,207 @@
+"""ctypes binding of the C ABI of libscess_gpu.so (include/scess_gpu.h).
+
+There is no CPU fallback: if the library is missing, or no GPU is present,
+every entry point raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libscess_gpu.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "scess_gpu.h")
+
+SCESS_OK = 0
+SCESS_ERR_ARG = -1
+SCESS_ERR_HIP = -2
+SCESS_ERR_OOM = -3
+SCESS_ERR_STATE = -4
+SCESS_CANCELLED = -5
+
+PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class TIn(ctypes.Structure):
+    _fields_ = [("o", ctypes.c_int64), ("v", ctypes.c_int64), ("e", _dp), ("t1", _dp), ("t2", _dp),
+                ("ovvv", _dp), ("ovoo", _dp), ("ovov", _dp)]
+
+
+class TStats(ctypes.Structure):
+    _fields_ = [("seconds", ctypes.c_double), ("w_build_ms", ctypes.c_double), ("epilogue_ms", ctypes.c_double),
+                ("w_build_launches", ctypes.c_int64), ("epilogue_launches", ctypes.c_int64),
+                ("n_ijk", ctypes.c_int64), ("flops_algorithmic", ctypes.c_double),
+                ("flops_executed", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class TGrad(ctypes.Structure):
+    _fields_ = [("ovvv", _dp), ("ovoo", _dp), ("ovov", _dp), ("t1", _dp), ("t2", _dp)]
+
+
+STOP_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p)
+
+_LIB = None
+
+
+class ScessError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"scess error {code}: {msg}")
+        self.code = code
+
+
+def header_functions() -> list[str]:
+    """Names of the functions declared in include/scess_gpu.h."""
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return re.findall(r"\b(scess_[a-z0-9_]+)\s*\(", txt)
+
+
+def load(path: str = LIB_PATH):
+    """Load libscess_gpu.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python -m scess_amd.build` "
+                           "(there is no CPU fallback)")
+    lib = ctypes.CDLL(path)
+    c = ctypes
+    lib.scess_create.argtypes = [c.POINTER(c.c_void_p), c.c_int]
+    lib.scess_create.restype = c.c_int
+    lib.scess_destroy.argtypes = [c.c_void_p]
+    lib.scess_destroy.restype = None
+    lib.scess_last_error.argtypes = [c.c_void_p]
+    lib.scess_last_error.restype = c.c_char_p
+    lib.scess_set_should_stop.argtypes = [c.c_void_p, STOP_FN, c.c_void_p]
+    lib.scess_set_should_stop.restype = c.c_int
+    lib.scess_set_batch.argtypes = [c.c_void_p, c.c_int64]
+    lib.scess_set_batch.restype = c.c_int
+    lib.scess_t_set_inputs.argtypes = [c.c_void_p, c.POINTER(TIn), c.c_int]
+    lib.scess_t_set_inputs.restype = c.c_int
+    lib.scess_t_forward.argtypes = [c.c_void_p, c.c_int, _dp, c.POINTER(TStats)]
+    lib.scess_t_forward.restype = c.c_int
+    lib.scess_t_forward_ex.argtypes = [c.c_void_p, c.c_int, _dp, _dp, _dp, c.POINTER(TStats)]
+    lib.scess_t_forward_ex.restype = c.c_int
+    lib.scess_t_num_ijk.argtypes = [c.c_void_p]
+    lib.scess_t_num_ijk.restype = c.c_int64
+    lib.scess_t_num_orbits.argtypes = [c.c_void_p]
+    lib.scess_t_num_orbits.restype = c.c_int64
+    if hasattr(lib, "scess_t_backward"):
+        lib.scess_t_backward.argtypes = [c.c_void_p, c.c_int, c.c_double, c.POINTER(TGrad)]
+        lib.scess_t_backward.restype = c.c_int
+    _LIB = lib
+    return lib
+
+
+def _f64(a) -> np.ndarray:
+    a = np.asarray(a, dtype=np.float64)
+    if a.ndim > 1 and not a.flags.f_contiguous:
+        a = np.asfortranarray(a)
+    elif a.ndim <= 1:
+        a = np.ascontiguousarray(a)
+    return a
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+class Context:
+    """One scess_ctx (one HIP stream on one device)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        h = ctypes.c_void_p()
+        rc = self.lib.scess_create(ctypes.byref(h), device)
+        if rc != SCESS_OK:
+            raise ScessError(rc, f"scess_create(device={device}) failed")
+        self.h = h
+        self._keep = None
+        self._stop = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.scess_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def check(self, rc):
+        if rc != SCESS_OK:
+            msg = self.lib.scess_last_error(self.h).decode(errors="replace")
+            raise ScessError(rc, msg)
+        return rc
+
+    def set_batch(self, n: int):
+        self.check(self.lib.scess_set_batch(self.h, int(n)))
+
+    def set_should_stop(self, fn):
+        """fn() -> bool, polled between batches of occupied triples."""
+        self._stop = STOP_FN(lambda _u: 1 if fn() else 0) if fn is not None else STOP_FN(0)
+        self.check(self.lib.scess_set_should_stop(self.h, self._stop, None))
+
+    def set_triples_inputs(self, inp, precisions=("fp64",)):
+        arrs = dict(e=_f64(inp.e), t1=_f64(inp.t1), t2=_f64(inp.t2), ovvv=_f64(inp.ovvv),
+                    ovoo=_f64(inp.ovoo), ovov=_f64(inp.ovov))
+        o, v = int(inp.o), int(inp.v)
+        expect = dict(e=(o + v,), t1=(o, v), t2=(o, o, v, v), ovvv=(o, v, v, v), ovoo=(o, v, o, o),
+                      ovov=(o, v, o, v))
+        for k, shp in expect.items():
+            if arrs[k].shape != shp:
+                raise ValueError(f"{k} has shape {arrs[k].shape}, expected {shp}")
+        tin = TIn(o, v, *[_ptr(arrs[k]) for k in ("e", "t1", "t2", "ovvv", "ovoo", "ovov")])
+        mask = 0
+        for p in precisions:
+            mask |= 1 << PRECISIONS[p]
+        self.check(self.lib.scess_t_set_inputs(self.h, ctypes.byref(tin), mask))
+        self.o, self.v = o, v
+
+    def triples_forward(self, precision="fp64", with_F=False, with_partials=False):
+        """E(T) [, F_ijk][, partials] and the stats dict."""
+        E = ctypes.c_double(0.0)
+        st = TStats()
+        F = P = None
+        if with_F:
+            F = np.zeros(self.lib.scess_t_num_ijk(self.h))
+        if with_partials:
+            P = np.zeros((self.lib.scess_t_num_ijk(self.h), self.lib.scess_t_num_orbits(self.h)))
+        rc = self.lib.scess_t_forward_ex(self.h, PRECISIONS[precision], ctypes.byref(E),
+                                         _ptr(F) if F is not None else None,
+                                         _ptr(P) if P is not None else None, ctypes.byref(st))
+        self.check(rc)
+        out = [E.value]
+        if with_F:
+            out.append(F)
+        if with_partials:
+            out.append(P)
+        out.append(st.as_dict())
+        return tuple(out)
+
+    def triples_backward(self, precision="fp64", dE=1.0):
+        """Gradients of dE * E(T) with respect to ovvv, ovoo, ovov, t1, t2 (input layouts)."""
+        o, v = self.o, self.v
+        g = {"ovvv": np.zeros((o, v, v, v), order="F"), "ovoo": np.zeros((o, v, o, o), order="F"),
+             "ovov": np.zeros((o, v, o, v), order="F"), "t1": np.zeros((o, v), order="F"),
+             "t2": np.zeros((o, o, v, v), order="F")}
+        tg = TGrad(*[_ptr(g[k]) for k in ("ovvv", "ovoo", "ovov", "t1", "t2")])
+        self.check(self.lib.scess_t_backward(self.h, PRECISIONS[precision], float(dE), ctypes.byref(tg)))
+        return g

@@ -1,0 +1,120 @@
+"""Parity of the gfx950 (T) path (C ABI) with the CPU oracle.
+
+Tolerances (stated per test): FP64 contractions agree with the FP64 oracle to
+1e-12 relative (different but deterministic summation orders); FP32 and
+split-BF16x3 contractions are compared with the FP64 GPU result.
+"""
+
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from scess_amd import gpu
+from scess_amd.synthetic import TriplesInputs, make_triples_inputs
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = gpu.Context(0)
+    yield c
+    c.close()
+
+
+def _rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+@pytest.mark.parametrize("o,v,seed", [(2, 3, 1), (3, 5, 2), (4, 7, 3), (5, 19, 4), (3, 17, 5), (6, 33, 6),
+                                      (8, 40, 7), (7, 70, 8)])
+def test_fp64_matches_oracle(ctx, o, v, seed):
+    _say(f"case fp64 o={o} v={v} seed={seed}")
+    inp = make_triples_inputs(o, v, seed)
+    ctx.set_triples_inputs(inp, ("fp64",))
+    E, F, st = ctx.triples_forward("fp64", with_F=True)
+    Eo = oracle.t_abc(inp)
+    _, Fo = oracle.t_ijk(inp)
+    assert _rel(E, Eo) < 1e-12, (E, Eo)
+    assert np.allclose(F, Fo, rtol=1e-11, atol=1e-13 * np.abs(Fo).max())
+
+
+def test_single_occupied_or_virtual_is_zero(ctx):
+    for (o, v) in [(1, 9), (4, 1)]:
+        _say(f"case zero o={o} v={v}")
+        inp = make_triples_inputs(o, v, 9)
+        ctx.set_triples_inputs(inp, ("fp64",))
+        E, _ = ctx.triples_forward("fp64")
+        assert abs(E) < 1e-14
+
+
+def test_deterministic(ctx):
+    _say("case deterministic o=6 v=40")
+    inp = make_triples_inputs(6, 40, 10)
+    ctx.set_triples_inputs(inp, ("fp64", "fp32", "bf16x3"))
+    for p in ("fp64", "fp32", "bf16x3"):
+        E1, _ = ctx.triples_forward(p)
+        ctx.set_batch(3)
+        E2, _ = ctx.triples_forward(p)
+        ctx.set_batch(0)
+        assert E1 == E2, p
+
+
+@pytest.mark.parametrize("o,v,seed", [(4, 20, 11), (8, 64, 12), (10, 100, 13)])
+def test_mixed_precision_close_to_fp64(ctx, o, v, seed):
+    _say(f"case mixed o={o} v={v}")
+    inp = make_triples_inputs(o, v, seed)
+    ctx.set_triples_inputs(inp, ("fp64", "fp32", "bf16x3"))
+    E64, _ = ctx.triples_forward("fp64")
+    E32, _ = ctx.triples_forward("fp32")
+    Eb3, _ = ctx.triples_forward("bf16x3")
+    _say(f"  E64={E64!r} rel32={_rel(E32, E64):.3e} relbf16x3={_rel(Eb3, E64):.3e}")
+    assert _rel(E32, E64) < 1e-5
+    assert _rel(Eb3, E64) < 1e-4
+
+
+def test_block_partials_sum_to_F(ctx):
+    _say("case partials o=5 v=30")
+    inp = make_triples_inputs(5, 30, 14)
+    ctx.set_triples_inputs(inp, ("fp64",))
+    E, F, P, _ = ctx.triples_forward("fp64", with_F=True, with_partials=True)
+    assert np.allclose(P.sum(axis=1), F, rtol=1e-13, atol=1e-13 * np.abs(F).max())
+
+
+def test_errors_and_cancellation():
+    _say("case errors")
+    c = gpu.Context(0)
+    with pytest.raises(gpu.ScessError) as ei:
+        c.triples_forward("fp64")
+    assert ei.value.code == gpu.SCESS_ERR_STATE
+    inp = make_triples_inputs(4, 12, 15)
+    c.set_triples_inputs(inp, ("fp64",))
+    with pytest.raises(gpu.ScessError) as ei:
+        c.triples_forward("fp32")
+    assert ei.value.code == gpu.SCESS_ERR_STATE
+    c.set_should_stop(lambda: True)
+    with pytest.raises(gpu.ScessError) as ei:
+        c.triples_forward("fp64")
+    assert ei.value.code == gpu.SCESS_CANCELLED
+    c.set_should_stop(None)
+    E, _ = c.triples_forward("fp64")
+    assert _rel(E, oracle.t_abc(inp)) < 1e-12
+    c.close()
+
+
+def test_rccsd_triplescorrection_full_eri_api():
+    """The n^4 <pq|rs> entry point (CC.h:154-158) slices the same blocks."""
+    _say("case full-eri api o=4 v=9")
+    from scess_amd.synthetic import full_eri_phys
+    from scess_amd.triples import RCCSD_TriplesCorrection
+    inp = make_triples_inputs(4, 9, 16, keep_factor=True)
+    eri = full_eri_phys(inp.L)
+    E = RCCSD_TriplesCorrection(eri, inp.e, 4, inp.t1, inp.t2)
+    assert _rel(E, oracle.t_abc(inp)) < 1e-12
