@@ -36,6 +36,12 @@ def _bind(lib):
     lib.oracle_t_abc.restype = ctypes.c_double
     lib.oracle_t_abc.argtypes = [ctypes.c_int64, ctypes.c_int64, _dp, _dp, _dp, _dp, _dp, _dp,
                                  _i64p, ctypes.c_int64, ctypes.c_int, _dp]
+    lib.oracle_t_abc_prepare.restype = ctypes.c_void_p
+    lib.oracle_t_abc_prepare.argtypes = [ctypes.c_int64, ctypes.c_int64, _dp, _dp, _dp, _dp, _dp, _dp]
+    lib.oracle_t_abc_eval.restype = ctypes.c_double
+    lib.oracle_t_abc_eval.argtypes = [ctypes.c_void_p, _i64p, ctypes.c_int64, ctypes.c_int, _dp]
+    lib.oracle_t_abc_free.restype = None
+    lib.oracle_t_abc_free.argtypes = [ctypes.c_void_p]
     lib.oracle_t_ijk.restype = ctypes.c_double
     lib.oracle_t_ijk.argtypes = [ctypes.c_int64, ctypes.c_int64, _dp, _dp, _dp, _dp, _dp, _dp,
                                  _i64p, ctypes.c_int64, ctypes.c_int, _dp]
@@ -101,6 +107,29 @@ def t_abc(inp, items=None, nthreads=None, item_energy=False, lib=None):
                          it.ctypes.data_as(_i64p) if it is not None else None,
                          0 if it is None else len(it), nthreads or num_threads(), _p(out))
     return (E, out[:n]) if item_energy else E
+
+
+class AbcPrepared:
+    """Repacked oracle inputs, for timing the per-triple work apart from the one-time repack."""
+
+    def __init__(self, inp, lib=None):
+        self.lib = lib or load()
+        self.arrs = _args(inp)
+        self.v = inp.v
+        self.h = self.lib.oracle_t_abc_prepare(inp.o, inp.v, *[_p(a) for a in self.arrs])
+
+    def eval(self, items=None, nthreads=None):
+        it = None if items is None else np.ascontiguousarray(items, dtype=np.int64)
+        return self.lib.oracle_t_abc_eval(self.h, it.ctypes.data_as(_i64p) if it is not None else None,
+                                          0 if it is None else len(it), nthreads or num_threads(), None)
+
+    def close(self):
+        if self.h:
+            self.lib.oracle_t_abc_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 def t_ijk(inp, items=None, nthreads=None, lib=None):

@@ -231,88 +231,113 @@ static void *abc_worker(void *arg) {
 
 int64_t oracle_num_unique_triples(int64_t v) { return v * (v + 1) * (v + 2) / 6; }
 
-/*
- * E(T) of the abc-driven restatement. items == NULL evaluates every unique
- * triple; otherwise only the listed ones (indices into the enumeration
- * a = 0..v-1, b = 0..a, c = 0..b of RCCSD_T.cpp:109-114), and the return value
- * is 2 * (sum over those triples). item_energy (optional) receives the
- * per-triple energies before the factor 2.
- */
-double oracle_t_abc(int64_t o, int64_t v, const double *e, const double *t1, const double *t2,
-                    const double *ovvv, const double *ovoo, const double *ovov, const int64_t *items,
-                    int64_t nitems, int nthreads, double *item_energy) {
-    const idx_t o2 = o * o, o3 = o2 * o;
+/* Repacked inputs of the abc-driven restatement (RCCSD_T.cpp:27-56), reusable across evaluations. */
+typedef struct {
     abc_job_t J;
-    memset(&J, 0, sizeof(J));
-    J.o = o;
-    J.v = v;
-    J.e = e;
-    J.t1 = t1;
-    J.t2 = t2;
-    J.P = (double *)malloc(sizeof(double) * (size_t)(o * v * v * v));
-    J.R = (double *)malloc(sizeof(double) * (size_t)(o3 * v));
-    J.S = (double *)malloc(sizeof(double) * (size_t)(o2 * v * v));
-    J.Q = (double *)malloc(sizeof(double) * (size_t)(v * o2 * v));
+    int32_t *trip;
+    idx_t ntrip;
+} abc_handle_t;
+
+void *oracle_t_abc_prepare(int64_t o, int64_t v, const double *e, const double *t1, const double *t2,
+                           const double *ovvv, const double *ovoo, const double *ovov) {
+    const idx_t o2 = o * o, o3 = o2 * o;
+    abc_handle_t *H = (abc_handle_t *)calloc(1, sizeof(abc_handle_t));
+    abc_job_t *J = &H->J;
+    J->o = o;
+    J->v = v;
+    J->e = e;
+    J->t1 = t1;
+    J->t2 = t2;
+    J->P = (double *)malloc(sizeof(double) * (size_t)(o * v * v * v));
+    J->R = (double *)malloc(sizeof(double) * (size_t)(o3 * v));
+    J->S = (double *)malloc(sizeof(double) * (size_t)(o2 * v * v));
+    J->Q = (double *)malloc(sizeof(double) * (size_t)(v * o2 * v));
     /* P(i,f,a,b) = (ia|bf), R(i,j,m,a) = (ia|mj), S(i,j,a,b) = (ia|jb), Q(f,j,k,c) = t_kj^cf (RCCSD_T.cpp:27-56) */
     for (idx_t b = 0; b < v; b++)
         for (idx_t a = 0; a < v; a++)
             for (idx_t f = 0; f < v; f++)
                 for (idx_t i = 0; i < o; i++)
-                    J.P[i + o * (f + v * (a + v * b))] = ovvv[i + o * (a + v * (b + v * f))];
+                    J->P[i + o * (f + v * (a + v * b))] = ovvv[i + o * (a + v * (b + v * f))];
     for (idx_t a = 0; a < v; a++)
         for (idx_t m = 0; m < o; m++)
             for (idx_t j = 0; j < o; j++)
                 for (idx_t i = 0; i < o; i++)
-                    J.R[i + o * (j + o * (m + o * a))] = ovoo[i + o * (a + v * (m + o * j))];
+                    J->R[i + o * (j + o * (m + o * a))] = ovoo[i + o * (a + v * (m + o * j))];
     for (idx_t b = 0; b < v; b++)
         for (idx_t a = 0; a < v; a++)
             for (idx_t j = 0; j < o; j++)
                 for (idx_t i = 0; i < o; i++)
-                    J.S[i + o * (j + o * (a + v * b))] = ovov[i + o * (a + v * (j + o * b))];
+                    J->S[i + o * (j + o * (a + v * b))] = ovov[i + o * (a + v * (j + o * b))];
     for (idx_t c = 0; c < v; c++)
         for (idx_t k = 0; k < o; k++)
             for (idx_t j = 0; j < o; j++)
                 for (idx_t f = 0; f < v; f++)
-                    J.Q[f + v * (j + o * (k + o * c))] = t2[k + o * (j + o * (c + v * f))];
-
-    const idx_t ntrip = oracle_num_unique_triples(v);
-    int32_t *trip = (int32_t *)malloc(sizeof(int32_t) * 3 * (size_t)(ntrip > 0 ? ntrip : 1));
+                    J->Q[f + v * (j + o * (k + o * c))] = t2[k + o * (j + o * (c + v * f))];
+    H->ntrip = oracle_num_unique_triples(v);
+    H->trip = (int32_t *)malloc(sizeof(int32_t) * 3 * (size_t)(H->ntrip > 0 ? H->ntrip : 1));
     idx_t t = 0;
     for (idx_t a = 0; a < v; a++)
         for (idx_t b = 0; b <= a; b++)
             for (idx_t c = 0; c <= b; c++) {
-                trip[3 * t] = (int32_t)a;
-                trip[3 * t + 1] = (int32_t)b;
-                trip[3 * t + 2] = (int32_t)c;
+                H->trip[3 * t] = (int32_t)a;
+                H->trip[3 * t + 1] = (int32_t)b;
+                H->trip[3 * t + 2] = (int32_t)c;
                 t++;
             }
-    J.trip = trip;
-    J.items = items;
-    J.nitems = items ? nitems : ntrip;
-    double *energies = item_energy ? item_energy : (double *)malloc(sizeof(double) * (size_t)(J.nitems + 1));
-    J.item_energy = energies;
-    atomic_init(&J.next, 0);
+    J->trip = H->trip;
+    return H;
+}
 
+void oracle_t_abc_free(void *h) {
+    abc_handle_t *H = (abc_handle_t *)h;
+    if (!H)
+        return;
+    free(H->J.P);
+    free(H->J.R);
+    free(H->J.S);
+    free(H->J.Q);
+    free(H->trip);
+    free(H);
+}
+
+/*
+ * E(T) of the abc-driven restatement from prepared inputs. items == NULL
+ * evaluates every unique triple; otherwise only the listed ones (indices into
+ * the enumeration a = 0..v-1, b = 0..a, c = 0..b of RCCSD_T.cpp:109-114), and
+ * the return value is 2 * (sum over those triples). item_energy (optional)
+ * receives the per-triple energies before the factor 2.
+ */
+double oracle_t_abc_eval(void *h, const int64_t *items, int64_t nitems, int nthreads, double *item_energy) {
+    abc_handle_t *H = (abc_handle_t *)h;
+    abc_job_t *J = &H->J;
+    J->items = items;
+    J->nitems = items ? nitems : H->ntrip;
+    double *energies = item_energy ? item_energy : (double *)malloc(sizeof(double) * (size_t)(J->nitems + 1));
+    J->item_energy = energies;
+    atomic_init(&J->next, 0);
     if (nthreads < 1)
         nthreads = 1;
-    if (nthreads > J.nitems)
-        nthreads = (int)(J.nitems > 0 ? J.nitems : 1);
+    if (nthreads > J->nitems)
+        nthreads = (int)(J->nitems > 0 ? J->nitems : 1);
     pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)nthreads);
     for (int w = 1; w < nthreads; w++)
-        pthread_create(&th[w], NULL, abc_worker, &J);
-    abc_worker(&J);
+        pthread_create(&th[w], NULL, abc_worker, J);
+    abc_worker(J);
     for (int w = 1; w < nthreads; w++)
         pthread_join(th[w], NULL);
     free(th);
-
-    const double E = 2.0 * nsum_array(energies, J.nitems);
+    const double E = 2.0 * nsum_array(energies, J->nitems);
     if (!item_energy)
         free(energies);
-    free(trip);
-    free(J.P);
-    free(J.R);
-    free(J.S);
-    free(J.Q);
+    return E;
+}
+
+double oracle_t_abc(int64_t o, int64_t v, const double *e, const double *t1, const double *t2,
+                    const double *ovvv, const double *ovoo, const double *ovov, const int64_t *items,
+                    int64_t nitems, int nthreads, double *item_energy) {
+    void *h = oracle_t_abc_prepare(o, v, e, t1, t2, ovvv, ovoo, ovov);
+    const double E = oracle_t_abc_eval(h, items, nitems, nthreads, item_energy);
+    oracle_t_abc_free(h);
     return E;
 }
 
