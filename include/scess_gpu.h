@@ -108,12 +108,21 @@ int scess_t_forward(scess_ctx *ctx, int prec, double *e_t, scess_t_stats *stats)
 /* As scess_t_forward, and optionally also:
  *   F_ijk     [scess_t_num_ijk] per unique occupied triple (enumeration i outer,
  *             j <= i, k <= j), F = sum_abc W r3(Y)/D before the 2/deg weight;
- *   partials  [scess_t_num_ijk * scess_t_num_orbits] per (ijk, virtual block
- *             orbit) pieces of F, for local finite differences. */
+ *   partials  [scess_t_num_ijk * scess_t_num_partials] pieces of F per ijk (each
+ *             the sum over a run of virtual-block orbits), for local finite
+ *             differences. */
 int scess_t_forward_ex(scess_ctx *ctx, int prec, double *e_t, double *F_ijk, double *partials,
                        scess_t_stats *stats);
 int64_t scess_t_num_ijk(const scess_ctx *ctx);
-int64_t scess_t_num_orbits(const scess_ctx *ctx);
+int64_t scess_t_num_partials(const scess_ctx *ctx);
+
+/* Adjoint of E(T) (no counterpart in the reference): writes dE * dE(T)/d(input)
+ * for every input entry, in the input layouts, into the caller's host arrays
+ * (overwritten). Every array entry is an independent variable read exactly as
+ * the forward reads it. W is recomputed per occupied triple in precision prec;
+ * the back-contractions run in FP64 and need SCESS_PREC_FP64 packed by
+ * scess_t_set_inputs. Deterministic: triples are processed in a fixed order. */
+int scess_t_backward(scess_ctx *ctx, int prec, double dE, scess_t_grad *grad);
 
 #ifdef __cplusplus
 }

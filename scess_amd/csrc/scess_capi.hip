@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -19,19 +20,21 @@ struct TriplesState {
     int o = 0, v = 0, vR = 0;
     bool have_prec[3] = {false, false, false};
     Operands ops[3];
-    double *S = nullptr, *t1 = nullptr, *e = nullptr;  // FP64 epilogue inputs
+    double *S = nullptr, *t1 = nullptr, *e = nullptr, *ev = nullptr;  // FP64 epilogue inputs
     std::vector<int> ijk_all;                          // 3 per unique occupied triple
     std::vector<int> work;                             // indices (into ijk_all) with i != k
     int *d_ijk_all = nullptr;
     std::vector<int> orbits;
     int *d_orbits = nullptr;
-    int nb = 0, norb = 0;
+    int nb = 0, norb = 0, ngrp = 0;  // blocks per dim, orbits, epilogue partials per ijk
     // work buffers
     void *W1 = nullptr, *W2 = nullptr;
     size_t wbytes = 0;
     int *d_work_ijk = nullptr, *d_work_index = nullptr;  // per work item (i != k)
     double *partials = nullptr;
     size_t partials_cap = 0;
+    double *U = nullptr;  // V pieces of one batch
+    size_t u_cap = 0;
     double *F = nullptr, *E = nullptr;
 };
 
@@ -84,6 +87,7 @@ void free_triples(TriplesState &t) {
     dfree(t.S);
     dfree(t.t1);
     dfree(t.e);
+    dfree(t.ev);
     dfree(t.d_ijk_all);
     dfree(t.d_orbits);
     dfree(t.W1);
@@ -91,6 +95,7 @@ void free_triples(TriplesState &t) {
     dfree(t.d_work_ijk);
     dfree(t.d_work_index);
     dfree(t.partials);
+    dfree(t.U);
     dfree(t.F);
     dfree(t.E);
     t = TriplesState{};
@@ -154,7 +159,7 @@ int scess_set_batch(scess_ctx *c, int64_t nbatch) {
 
 int64_t scess_t_num_ijk(const scess_ctx *c) { return c && c->t.have ? (int64_t)c->t.ijk_all.size() / 3 : 0; }
 
-int64_t scess_t_num_orbits(const scess_ctx *c) { return c && c->t.have ? c->t.norb : 0; }
+int64_t scess_t_num_partials(const scess_ctx *c) { return c && c->t.have ? c->t.ngrp : 0; }
 
 int scess_t_set_inputs(scess_ctx *c, const scess_t_in *in, int prec_mask) {
     if (!c)
@@ -191,12 +196,16 @@ int scess_t_set_inputs(scess_ctx *c, const scess_t_in *in, int prec_mask) {
         free_triples(t);
         return rc;
     }
-    // t1 as [o][v] (ColMajor (i,a) -> row-major i)
-    std::vector<double> t1r((size_t)(o * v));
+    // t1 as [o][vR] (ColMajor (i,a) -> row-major i, zero-padded rows) and the virtual energies padded to vR
+    const int64_t vRp = t.vR;
+    std::vector<double> t1r((size_t)(o * vRp), 0.0), evr((size_t)vRp, 0.0);
     for (int64_t i = 0; i < o; i++)
         for (int64_t a = 0; a < v; a++)
-            t1r[(size_t)(i * v + a)] = in->t1[i + o * a];
-    if ((rc = upload(c, t1r.data(), t1r.size(), &t.t1)) || (rc = upload(c, in->e, (size_t)(o + v), &t.e))) {
+            t1r[(size_t)(i * vRp + a)] = in->t1[i + o * a];
+    for (int64_t a = 0; a < v; a++)
+        evr[(size_t)a] = in->e[o + a];
+    if ((rc = upload(c, t1r.data(), t1r.size(), &t.t1)) || (rc = upload(c, in->e, (size_t)(o + v), &t.e)) ||
+        (rc = upload(c, evr.data(), evr.size(), &t.ev))) {
         cleanup();
         free_triples(t);
         return rc;
@@ -261,6 +270,7 @@ int scess_t_set_inputs(scess_ctx *c, const scess_t_in *in, int prec_mask) {
                 t.orbits.push_back(C);
             }
     t.norb = (int)(t.orbits.size() / 3);
+    t.ngrp = epilogue_groups(t.norb);
     HIP_TRY(c, hipMalloc((void **)&t.d_ijk_all, sizeof(int) * t.ijk_all.size()));
     HIP_TRY(c, hipMemcpy(t.d_ijk_all, t.ijk_all.data(), sizeof(int) * t.ijk_all.size(), hipMemcpyHostToDevice));
     HIP_TRY(c, hipMalloc((void **)&t.d_orbits, sizeof(int) * t.orbits.size()));
@@ -309,7 +319,7 @@ int scess_t_forward_ex(scess_ctx *c, int prec, double *e_t, double *F_ijk, doubl
     const int64_t nall = (int64_t)t.ijk_all.size() / 3;
     const int64_t nwork = (int64_t)t.work.size();
     const size_t accb = prec == P64 ? 8 : 4;
-    const int64_t wstride = v * vR * vR;
+    const int64_t wstride = vR * vR * vR;  // W in 8x8x8 blocks over vR^3
 
     // batch size: bounded W footprint (2 buffers), at least 1
     int64_t nbatch = c->nbatch_req;
@@ -330,16 +340,27 @@ int scess_t_forward_ex(scess_ctx *c, int prec, double *e_t, double *F_ijk, doubl
         t.wbytes = wneed;
     }
     // per-orbit partials: whole job if the caller wants them, else one batch
-    const size_t pneed = (size_t)(partials_out ? std::max<int64_t>(nwork, 1) : nbatch) * (size_t)t.norb;
+    const size_t pneed = (size_t)(partials_out ? std::max<int64_t>(nwork, 1) : nbatch) * (size_t)t.ngrp;
     if (t.partials_cap < pneed) {
         dfree(t.partials);
         HIP_TRY(c, hipMalloc((void **)&t.partials, sizeof(double) * pneed));
         t.partials_cap = pneed;
     }
+    const size_t uneed = (size_t)nbatch * 3 * (size_t)(v * vR);
+    if (t.u_cap < uneed) {
+        dfree(t.U);
+        HIP_TRY(c, hipMalloc((void **)&t.U, sizeof(double) * uneed));
+        t.u_cap = uneed;
+    }
     HIP_TRY(c, hipMemsetAsync(t.F, 0, sizeof(double) * nall, c->stream));
 
     const Operands &op = t.ops[prec];
-    const int blk = wbuild_tile(prec, (int)v);
+    int blk = wbuild_tile(prec, (int)v);
+    if (const char *ov = std::getenv("SCESS_WBUILD_BLK")) {  // tuning override: 1, 2, 4 or 5
+        const int b = std::atoi(ov);
+        if (b == 1 || b == 2 || b == 4 || b == 5)
+            blk = b;
+    }
     const int BM = 2 * blk * 16;
     WBuildArgs wa;
     wa.X = (const char *)op.X;
@@ -369,9 +390,11 @@ int scess_t_forward_ex(scess_ctx *c, int prec, double *e_t, double *F_ijk, doubl
     ea.nb = t.nb;
     ea.orbits = t.d_orbits;
     ea.norb = t.norb;
-    ea.S = t.S;
+    ea.ngrp = t.ngrp;
     ea.t1 = t.t1;
     ea.e = t.e;
+    ea.ev = t.ev;
+    ea.U = t.U;
 
     std::vector<hipEvent_t> ev;
     auto new_event = [&](hipEvent_t *e) -> hipError_t {
@@ -417,18 +440,20 @@ int scess_t_forward_ex(scess_ctx *c, int prec, double *e_t, double *F_ijk, doubl
             rc = fail(c, SCESS_ERR_HIP, "hipEventCreate");
             break;
         }
-        double *pbase = t.partials + (partials_out ? (size_t)b0 * t.norb : 0);
+        double *pbase = t.partials + (partials_out ? (size_t)b0 * t.ngrp : 0);
         ea.partials = pbase;
         (void)hipEventRecord(be.a, c->stream);
         he = launch_wbuild(prec, blk, wa, nb, c->stream);
         if (he == hipSuccess)
             he = hipEventRecord(be.b, c->stream);
         if (he == hipSuccess)
+            he = launch_vpieces(t.S, ea.ijk, (int)o, (int)v, (int)vR, t.U, nb, c->stream);
+        if (he == hipSuccess)
             he = launch_epilogue(prec, ea, nb, c->stream);
         if (he == hipSuccess)
             he = hipEventRecord(be.c, c->stream);
         if (he == hipSuccess)
-            he = launch_reduce_partials(pbase, t.norb, t.d_work_index + b0, t.F, nb, c->stream);
+            he = launch_reduce_partials(pbase, t.ngrp, t.d_work_index + b0, t.F, nb, c->stream);
         if (he != hipSuccess) {
             rc = hip_fail(c, he, "kernel launch");
             break;
@@ -453,12 +478,12 @@ int scess_t_forward_ex(scess_ctx *c, int prec, double *e_t, double *F_ijk, doubl
             he = hipMemcpy(F_ijk, t.F, sizeof(double) * nall, hipMemcpyDeviceToHost);
         if (he == hipSuccess && partials_out) {
             // scatter per-work-item partials to the full unique-ijk enumeration (zeros for i == j == k)
-            std::memset(partials_out, 0, sizeof(double) * (size_t)nall * t.norb);
-            std::vector<double> hp((size_t)nwork * t.norb);
+            std::memset(partials_out, 0, sizeof(double) * (size_t)nall * t.ngrp);
+            std::vector<double> hp((size_t)nwork * t.ngrp);
             he = hipMemcpy(hp.data(), t.partials, sizeof(double) * hp.size(), hipMemcpyDeviceToHost);
             for (int64_t w = 0; he == hipSuccess && w < nwork; w++)
-                std::memcpy(partials_out + (size_t)t.work[(size_t)w] * t.norb, hp.data() + (size_t)w * t.norb,
-                            sizeof(double) * t.norb);
+                std::memcpy(partials_out + (size_t)t.work[(size_t)w] * t.ngrp, hp.data() + (size_t)w * t.ngrp,
+                            sizeof(double) * t.ngrp);
         }
         if (he != hipSuccess)
             rc = hip_fail(c, he, "result copy");
@@ -486,6 +511,269 @@ int scess_t_forward_ex(scess_ctx *c, int prec, double *e_t, double *F_ijk, doubl
                                 (6.0 * (double)op.vK + 6.0 * (double)op.oK);
     }
     destroy_events();
+    return rc;
+}
+
+
+int scess_t_backward(scess_ctx *c, int prec, double dE, scess_t_grad *grad) {
+    if (!c)
+        return SCESS_ERR_ARG;
+    if (!grad || !grad->ovvv || !grad->ovoo || !grad->ovov || !grad->t1 || !grad->t2)
+        return fail(c, SCESS_ERR_ARG, "scess_t_backward: null gradient array");
+    if (prec < 0 || prec > 2)
+        return fail(c, SCESS_ERR_ARG, "scess_t_backward: unknown precision");
+    TriplesState &t = c->t;
+    if (!t.have)
+        return fail(c, SCESS_ERR_STATE, "scess_t_backward: no inputs (call scess_t_set_inputs first)");
+    if (!t.have_prec[prec] || !t.have_prec[P64])
+        return fail(c, SCESS_ERR_STATE, "scess_t_backward: needs SCESS_PREC_FP64 (and prec) packed");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const int64_t o = t.o, v = t.v, vR = t.vR;
+    const Operands &op = t.ops[prec];
+    const Operands &o64 = t.ops[P64];
+    const int64_t vK = o64.vK, oK = o64.oK;
+    const size_t accb = prec == P64 ? 8 : 4;
+    const int64_t wstride = vR * vR * vR;
+
+    // device buffers of this call, freed on every exit
+    std::vector<void *> bufs;
+    auto alloc = [&](size_t bytes, void **p) -> hipError_t {
+        hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 16));
+        if (e == hipSuccess) {
+            bufs.push_back(*p);
+            e = hipMemsetAsync(*p, 0, std::max<size_t>(bytes, 16), c->stream);
+        }
+        return e;
+    };
+    auto release = [&]() {
+        (void)hipStreamSynchronize(c->stream);
+        for (void *p : bufs)
+            (void)hipFree(p);
+        bufs.clear();
+    };
+    const int PT = 16, PR = 32;  // K splits of dT (K = v^2) and dRn (K = v^2)
+    double *dX, *dT, *dRn, *dTM, *dS, *dt1, *GW, *GV, *U, *dU, *dtv, *scr, *out;
+    void *W1, *W2;
+    hipError_t he = hipSuccess;
+    const size_t nX = (size_t)(o * v * v * vK), nT = (size_t)(o * o * v * vK), nR = (size_t)(o * o * v * oK),
+                 nTM = (size_t)(o * v * v * oK), nS = (size_t)(o * o * v * v), nt1 = (size_t)(o * vR),
+                 nG = (size_t)wstride, nscr = (size_t)std::max<int64_t>(PT * v * v, PR * v * o);
+    const size_t nout = std::max<size_t>({(size_t)(o * v * v * v), (size_t)(o * o * v * v), (size_t)(o * v * o * v)});
+    if ((he = alloc(8 * nX, (void **)&dX)) != hipSuccess || (he = alloc(8 * nT, (void **)&dT)) != hipSuccess ||
+        (he = alloc(8 * nR, (void **)&dRn)) != hipSuccess || (he = alloc(8 * nTM, (void **)&dTM)) != hipSuccess ||
+        (he = alloc(8 * nS, (void **)&dS)) != hipSuccess || (he = alloc(8 * nt1, (void **)&dt1)) != hipSuccess ||
+        (he = alloc(8 * nG, (void **)&GW)) != hipSuccess || (he = alloc(8 * nG, (void **)&GV)) != hipSuccess ||
+        (he = alloc(8 * 3 * (size_t)(v * vR), (void **)&U)) != hipSuccess ||
+        (he = alloc(8 * 3 * (size_t)(v * v), (void **)&dU)) != hipSuccess ||
+        (he = alloc(8 * 3 * (size_t)v, (void **)&dtv)) != hipSuccess ||
+        (he = alloc(8 * nscr, (void **)&scr)) != hipSuccess || (he = alloc(8 * nout, (void **)&out)) != hipSuccess ||
+        (he = alloc(accb * (size_t)wstride, &W1)) != hipSuccess ||
+        (he = alloc(accb * (size_t)wstride, &W2)) != hipSuccess) {
+        release();
+        return hip_fail(c, he, "scess_t_backward: device allocation");
+    }
+
+    const int blk = wbuild_tile(prec, (int)v);
+    const int BM = 2 * blk * 16;
+    WBuildArgs wa;
+    wa.X = (const char *)op.X;
+    wa.T = (const char *)op.T;
+    wa.Rn = (const char *)op.Rn;
+    wa.TM = (const char *)op.TM;
+    wa.o = (int)o;
+    wa.v = (int)v;
+    wa.vR = (int)vR;
+    wa.rbv = op.rbv;
+    wa.rbo = op.rbo;
+    wa.nchv = (int)(op.vK / kChunk[prec]);
+    wa.ncho = (int)(op.oK / kChunk[prec]);
+    wa.nt = (int)((v + BM - 1) / BM);
+    wa.ntile1 = (int)(v * wa.nt * wa.nt);
+    wa.W1 = W1;
+    wa.W2 = W2;
+    wa.wstride = wstride;
+    EpiArgs ea{};
+    ea.W1 = W1;
+    ea.W2 = W2;
+    ea.wstride = wstride;
+    ea.o = (int)o;
+    ea.v = (int)v;
+    ea.vR = (int)vR;
+    ea.nb = t.nb;
+    ea.orbits = t.d_orbits;
+    ea.norb = t.norb;
+    ea.ngrp = t.ngrp;
+    ea.t1 = t.t1;
+    ea.e = t.e;
+    ea.ev = t.ev;
+    ea.U = U;
+
+    static const int PERM[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+    const int64_t pos_stride[3] = {vR * vR, vR, 1};
+    const double *X64 = (const double *)o64.X, *T64 = (const double *)o64.T, *Rn64 = (const double *)o64.Rn,
+                 *TM64 = (const double *)o64.TM;
+    int rc = SCESS_OK;
+    bool cancelled = false;
+    const int64_t nwork = (int64_t)t.work.size();
+    for (int64_t w = 0; w < nwork && rc == SCESS_OK; w++) {
+        if ((w & 7) == 0) {
+            if ((he = hipStreamSynchronize(c->stream)) != hipSuccess) {
+                rc = hip_fail(c, he, "scess_t_backward");
+                break;
+            }
+            if (c->should_stop && c->should_stop(c->user)) {
+                cancelled = true;
+                break;
+            }
+        }
+        const int wi = t.work[(size_t)w];
+        const int occ[3] = {t.ijk_all[3 * wi], t.ijk_all[3 * wi + 1], t.ijk_all[3 * wi + 2]};
+        const double deg = (occ[0] == occ[1] || occ[1] == occ[2]) ? 2.0 : 1.0;
+        const int *dijk = t.d_work_ijk + 3 * w;
+        wa.ijk = dijk;
+        ea.ijk = dijk;
+        he = launch_wbuild(prec, blk, wa, 1, c->stream);
+        if (he == hipSuccess)
+            he = launch_vpieces(t.S, dijk, (int)o, (int)v, (int)vR, U, 1, c->stream);
+        if (he == hipSuccess)
+            he = launch_adjoint_g(prec, ea, dE * 2.0 / deg, GW, GV, c->stream);
+        for (int sgm = 0; sgm < 6 && he == hipSuccess; sgm++) {
+            const int64_t p = occ[PERM[sgm][0]], q = occ[PERM[sgm][1]], r = occ[PERM[sgm][2]];
+            const int64_t sx = pos_stride[PERM[sgm][0]], sy = pos_stride[PERM[sgm][1]], sz = pos_stride[PERM[sgm][2]];
+            // dX[p][x][y][f] += sum_z G(x,y,z) T[r][q][z][f]
+            {
+                GemmArgs g{};
+                g.nseg = 1;
+                g.M = (int)v;
+                g.N = (int)v;
+                g.alpha = 1.0;
+                g.splits = 1;
+                GemmSeg &sg = g.seg[0];
+                sg.K1 = 1;
+                sg.K2 = (int)v;
+                sg.A = {GW, 0, 0, sz, 0};
+                sg.B = {T64 + (r * o + q) * v * vK, 1, 0, vK, 0};
+                g.C = dX + p * v * v * vK;
+                g.sCn = 1;
+                if (sx == 1) {  // batch y, rows x
+                    sg.A.sb = sy;
+                    sg.A.sr = sx;
+                    g.sCb = vK;
+                    g.sCm = v * vK;
+                } else {  // batch x, rows y
+                    sg.A.sb = sx;
+                    sg.A.sr = sy;
+                    g.sCb = v * vK;
+                    g.sCm = vK;
+                }
+                he = launch_gemm(g, (int)v, c->stream);
+            }
+            // dT[r][q][z][f] += sum_{x,y} G(x,y,z) X[p][x][y][f]
+            if (he == hipSuccess) {
+                GemmArgs g{};
+                g.nseg = 1;
+                g.M = (int)v;
+                g.N = (int)v;
+                g.alpha = 1.0;
+                g.splits = PT;
+                g.scratch = scr;
+                GemmSeg &sg = g.seg[0];
+                sg.K1 = (int)v;
+                sg.K2 = (int)v;
+                if (sx < sy) {  // inner K over x
+                    sg.A = {GW, sz, sy, sx, 0};
+                    sg.B = {X64 + p * v * v * vK, 1, vK, v * vK, 0};
+                } else {  // inner K over y
+                    sg.A = {GW, sz, sx, sy, 0};
+                    sg.B = {X64 + p * v * v * vK, 1, v * vK, vK, 0};
+                }
+                g.C = dT + (r * o + q) * v * vK;
+                g.sCm = vK;
+                g.sCn = 1;
+                he = launch_gemm(g, 1, c->stream);
+            }
+            // dRn[p][q][x][m] += sum_{y,z} G(x,y,z) TM[r][y][z][m]
+            if (he == hipSuccess) {
+                GemmArgs g{};
+                g.nseg = 1;
+                g.M = (int)v;
+                g.N = (int)o;
+                g.alpha = 1.0;
+                g.splits = PR;
+                g.scratch = scr;
+                GemmSeg &sg = g.seg[0];
+                sg.K1 = (int)v;
+                sg.K2 = (int)v;
+                if (sy < sz) {  // inner K over y
+                    sg.A = {GW, sx, sz, sy, 0};
+                    sg.B = {TM64 + r * v * v * oK, 1, oK, v * oK, 0};
+                } else {  // inner K over z
+                    sg.A = {GW, sx, sy, sz, 0};
+                    sg.B = {TM64 + r * v * v * oK, 1, v * oK, oK, 0};
+                }
+                g.C = dRn + (p * o + q) * v * oK;
+                g.sCm = oK;
+                g.sCn = 1;
+                he = launch_gemm(g, 1, c->stream);
+            }
+            // dTM[r][y][z][m] += sum_x Rn[p][q][x][m] G(x,y,z)
+            if (he == hipSuccess) {
+                GemmArgs g{};
+                g.nseg = 1;
+                g.M = (int)v;
+                g.N = (int)o;
+                g.alpha = 1.0;
+                g.splits = 1;
+                GemmSeg &sg = g.seg[0];
+                sg.K1 = 1;
+                sg.K2 = (int)v;
+                sg.B = {Rn64 + (p * o + q) * v * oK, 1, 0, oK, 0};
+                g.C = dTM + r * v * v * oK;
+                g.sCn = 1;
+                if (sy == 1) {  // batch z, rows y
+                    sg.A = {GW, sy, 0, sx, sz};
+                    g.sCb = oK;
+                    g.sCm = v * oK;
+                } else {  // batch y, rows z
+                    sg.A = {GW, sz, 0, sx, sy};
+                    g.sCb = v * oK;
+                    g.sCm = oK;
+                }
+                he = launch_gemm(g, (int)v, c->stream);
+            }
+        }
+        if (he == hipSuccess)
+            he = launch_vback(GV, U, t.t1, occ[0], occ[1], occ[2], (int)o, (int)v, (int)vR, dU, dtv, dS, dt1,
+                              c->stream);
+        if (he != hipSuccess)
+            rc = hip_fail(c, he, "scess_t_backward: kernel launch");
+    }
+    if (rc == SCESS_OK && !cancelled) {
+        struct {
+            int mode;
+            const double *A, *B;
+            double *host;
+            size_t n;
+        } outs[5] = {{0, dX, nullptr, grad->ovvv, (size_t)(o * v * v * v)},
+                     {1, dRn, nullptr, grad->ovoo, (size_t)(o * v * o * o)},
+                     {2, dS, nullptr, grad->ovov, (size_t)(o * v * o * v)},
+                     {3, dT, dTM, grad->t2, (size_t)(o * o * v * v)},
+                     {4, dt1, nullptr, grad->t1, (size_t)(o * v)}};
+        for (auto &u : outs) {
+            he = launch_unpack(u.mode, u.A, u.B, out, (int)o, (int)v, vK, oK, vR, (int64_t)u.n, c->stream);
+            if (he == hipSuccess)
+                he = hipMemcpyAsync(u.host, out, 8 * u.n, hipMemcpyDeviceToHost, c->stream);
+            if (he == hipSuccess)
+                he = hipStreamSynchronize(c->stream);
+            if (he != hipSuccess) {
+                rc = hip_fail(c, he, "scess_t_backward: unpack");
+                break;
+            }
+        }
+    }
+    release();
+    if (rc == SCESS_OK && cancelled)
+        rc = fail(c, SCESS_CANCELLED, "scess_t_backward: cancelled");
     return rc;
 }
 

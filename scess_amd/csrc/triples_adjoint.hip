@@ -1,0 +1,467 @@
+// gfx950 kernels of the (T) adjoint: dE(T)/d(ovvv), d(ovoo), d(ovov), dt1, dt2.
+//
+// E = sum_ijk w_ijk sum_abc W r3(Y)/D with Y = W + V and w_ijk = 2/deg_ijk. Since
+// r3 is symmetric,  G_W = dE/dW = w (r3(Y)/D + r3(W/D))  and  G_V = dE/dV = w r3(W/D).
+// W is recomputed per ijk (the forward W-build), G is formed by an orbit kernel
+// like the forward epilogue, and the six permutation terms
+//   w(x,y,z; p,q,r) = sum_f X[p][x][y][f] T[r][q][z][f] + sum_m Rn[p][q][x][m] TM[r][y][z][m]
+// are back-contracted by one generic FP64 MFMA GEMM:
+//   dX[p][x][y][f]  += sum_z     G(x,y,z) T[r][q][z][f]
+//   dT[r][q][z][f]  += sum_{x,y} G(x,y,z) X[p][x][y][f]
+//   dRn[p][q][x][m] += sum_{y,z} G(x,y,z) TM[r][y][z][m]
+//   dTM[r][y][z][m] += sum_x     Rn[p][q][x][m] G(x,y,z)
+// Occupied triples are processed one after another and every kernel writes each
+// output element from one thread in a fixed order (split-K partials are summed
+// in split order), so the gradients are deterministic.
+#include "triples_internal.h"
+
+#include <cstdint>
+
+namespace scess {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+__device__ inline void nadd_a(double &s, double &c, double x) {
+    const double t = s + x;
+    if (fabs(s) >= fabs(x))
+        c += (s - t) + x;
+    else
+        c += (x - t) + s;
+    s = t;
+}
+
+// ------------------------------------------------------------------ G kernel
+// Orbit geometry as in the forward epilogue (8-wide virtual blocks).
+struct OrbitRowA {
+    bool own;
+    int ga, gb, gc0;
+    int off[5], step[5];
+};
+
+template <int PB, int PX, int PY>
+__device__ inline OrbitRowA orbit_row_a(const int *orbits, int orb, int tid) {
+    constexpr int EB = 8;
+    OrbitRowA r;
+    const int A = orbits[3 * orb], B = orbits[3 * orb + 1], C = orbits[3 * orb + 2];
+    const int bl1 = B != A ? B : C;
+    auto slot = [&](int blk) { return blk == A ? 0 : (blk == bl1 ? 1 : 2); };
+    const int sB = slot(B), sC = slot(C);
+    uint64_t map_lo = 0, map_hi = 0;
+    uint32_t seen = 0;
+    int nd = 0;
+    const int d = tid >> 6, ap = (tid >> 3) & 7, bp = tid & 7;
+    int P = 0, Q = 0, R = 0, sP = 0, sQ = 0, sR = 0;
+    auto add = [&](int p0, int p1, int p2, int s0, int s1, int s2) {
+        const int key = s0 * 9 + s1 * 3 + s2;
+        if (!(seen >> key & 1u)) {
+            seen |= 1u << key;
+            if (key < 21)
+                map_lo |= (uint64_t)nd << (3 * key);
+            else
+                map_hi |= (uint64_t)nd << (3 * (key - 21));
+            if (nd == d) {
+                P = p0;
+                Q = p1;
+                R = p2;
+                sP = s0;
+                sQ = s1;
+                sR = s2;
+            }
+            nd++;
+        }
+    };
+    add(A, B, C, 0, sB, sC);
+    add(B, C, A, sB, sC, 0);
+    add(C, A, B, sC, 0, sB);
+    add(C, B, A, sC, sB, 0);
+    add(A, C, B, 0, sC, sB);
+    add(B, A, C, sB, 0, sC);
+    auto dindex = [&](int s0, int s1, int s2) -> int {
+        const int key = s0 * 9 + s1 * 3 + s2;
+        return (int)((key < 21 ? map_lo >> (3 * key) : map_hi >> (3 * (key - 21))) & 7u);
+    };
+    r.own = d < nd;
+    r.ga = P * EB + ap;
+    r.gb = Q * EB + bp;
+    r.gc0 = R * EB;
+    r.off[0] = dindex(sQ, sR, sP) * PB + bp * PX + ap;  // (b, c, a)
+    r.step[0] = PY;
+    r.off[1] = dindex(sR, sP, sQ) * PB + ap * PY + bp;  // (c, a, b)
+    r.step[1] = PX;
+    r.off[2] = dindex(sR, sQ, sP) * PB + bp * PY + ap;  // (c, b, a)
+    r.step[2] = PX;
+    r.off[3] = dindex(sP, sR, sQ) * PB + ap * PX + bp;  // (a, c, b)
+    r.step[3] = PY;
+    r.off[4] = dindex(sQ, sP, sR) * PB + bp * PX + ap * PY;  // (b, a, c)
+    r.step[4] = 1;
+    return r;
+}
+
+constexpr int GK_THREADS = 384;
+
+// G_W and G_V (row-major [vR][vR][vR], FP64) of one occupied triple.
+template <typename AccT>
+__global__ __launch_bounds__(GK_THREADS) void adjoint_g_kernel(EpiArgs a, double wgt, double *GW, double *GV) {
+    constexpr int EB = 8;
+    constexpr int PY = 9, PX = EB * PY + 1, PB = EB * PX;
+    __shared__ double Yl[6 * PB];
+    __shared__ double Zl[6 * PB];
+    const int tid = threadIdx.x;
+    const int orb = blockIdx.x;
+    const int *q = a.ijk;
+    const int i = q[0], j = q[1], k = q[2];
+    const int v = a.v, vR = a.vR, nbR = a.vR >> 3;
+    const int64_t vr = (int64_t)v * vR;
+    const double eocc = a.e[i] + a.e[j] + a.e[k];
+    const double *U = a.U;
+    const AccT *W1b = (const AccT *)a.W1;
+    const AccT *W2b = (const AccT *)a.W2;
+    const int ap = (tid >> 3) & 7, bp = tid & 7, d = tid >> 6;
+
+    const OrbitRowA r = orbit_row_a<PB, PX, PY>(a.orbits, orb, tid);
+    double Y0[EB], Z0[EB], D0[EB];
+    const bool rowok = r.own && r.ga < v && r.gb < v;
+    if (rowok) {
+        const int P = r.ga >> 3, Q = r.gb >> 3, R = r.gc0 >> 3;
+        const AccT *W1 = W1b + (((int64_t)P * nbR + Q) * nbR + R) * 512 + (r.ga & 7) * 64 + (r.gb & 7) * 8;
+        const AccT *W2 = W2b + (((int64_t)Q * nbR + P) * nbR + R) * 512 + (r.gb & 7) * 64 + (r.ga & 7) * 8;
+        const double u0 = U[(int64_t)r.ga * vR + r.gb];
+        const double *u1 = U + vr + (int64_t)r.ga * vR + r.gc0;
+        const double *u2 = U + 2 * vr + (int64_t)r.gb * vR + r.gc0;
+        const double *tk = a.t1 + (int64_t)k * vR + r.gc0;
+        const double ti = a.t1[(int64_t)i * vR + r.ga], tj = a.t1[(int64_t)j * vR + r.gb];
+        const double eab = eocc - a.ev[r.ga] - a.ev[r.gb];
+#pragma unroll
+        for (int cp = 0; cp < EB; cp++) {
+            const bool ok = r.gc0 + cp < v;
+            const double w = (double)W1[cp] + (double)W2[cp];
+            const double Dv = eab - a.ev[r.gc0 + cp];
+            Y0[cp] = ok ? w + u0 * tk[cp] + u1[cp] * tj + u2[cp] * ti : 0.0;
+            Z0[cp] = ok ? w / Dv : 0.0;
+            D0[cp] = ok ? Dv : 1.0;
+        }
+    } else {
+#pragma unroll
+        for (int cp = 0; cp < EB; cp++) {
+            Y0[cp] = Z0[cp] = 0.0;
+            D0[cp] = 1.0;
+        }
+    }
+    if (r.own)
+#pragma unroll
+        for (int cp = 0; cp < EB; cp++) {
+            Yl[d * PB + ap * PX + bp * PY + cp] = Y0[cp];
+            Zl[d * PB + ap * PX + bp * PY + cp] = Z0[cp];
+        }
+    __syncthreads();
+    if (rowok) {
+        double *gw = GW + ((int64_t)r.ga * vR + r.gb) * vR + r.gc0;
+        double *gv = GV + ((int64_t)r.ga * vR + r.gb) * vR + r.gc0;
+#pragma unroll
+        for (int cp = 0; cp < EB; cp++) {
+            double y3 = 0.0, z3 = 0.0;
+            if (r.gc0 + cp < v) {
+                y3 = 4.0 * Y0[cp] + Yl[r.off[0] + cp * r.step[0]] + Yl[r.off[1] + cp * r.step[1]] -
+                     2.0 * (Yl[r.off[2] + cp * r.step[2]] + Yl[r.off[3] + cp * r.step[3]] + Yl[r.off[4] + cp * r.step[4]]);
+                z3 = 4.0 * Z0[cp] + Zl[r.off[0] + cp * r.step[0]] + Zl[r.off[1] + cp * r.step[1]] -
+                     2.0 * (Zl[r.off[2] + cp * r.step[2]] + Zl[r.off[3] + cp * r.step[3]] + Zl[r.off[4] + cp * r.step[4]]);
+            }
+            gw[cp] = wgt * (y3 / D0[cp] + z3);
+            gv[cp] = wgt * z3;
+        }
+    }
+}
+
+hipError_t launch_adjoint_g(int prec, const EpiArgs &a, double wgt, double *GW, double *GV, hipStream_t s) {
+    if (prec == P64)
+        hipLaunchKernelGGL(adjoint_g_kernel<double>, dim3((unsigned)a.norb), dim3(GK_THREADS), 0, s, a, wgt, GW, GV);
+    else
+        hipLaunchKernelGGL(adjoint_g_kernel<float>, dim3((unsigned)a.norb), dim3(GK_THREADS), 0, s, a, wgt, GW, GV);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------ generic GEMM
+// C[beta][m][n] += alpha * sum_seg sum_{k1,k2} A(m, k1, k2) B(n, k1, k2), FP64 MFMA 16x16x4,
+// 64x64 tiles (2x2 waves of 2x2 blocks), K chunks of 16 along k2 staged through LDS.
+// With splits > 1 (no batch), split p handles a fixed contiguous range of the K chunks
+// and writes its partial tile to scratch[p]; gemm_reduce_kernel adds them in split order.
+constexpr int GT = 64, GKC = 16, GPITCH = GT + 1;
+
+__device__ inline void gemm_load_tile(const GemmOp &op, int64_t base, int r0, int rext, int k1, int k2c, int K2,
+                                      double *lds, int tid) {
+    // 64 rows x 16 k, 4 elements per thread along the contiguous dimension
+    const bool kinner = op.s2 == 1 || (op.sr != 1 && (op.s2 < op.sr));
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        int rr, kk;
+        if (kinner) {
+            rr = tid >> 2;
+            kk = (tid & 3) * 4 + u;
+        } else {
+            kk = tid >> 4;
+            rr = (tid & 15) * 4 + u;
+        }
+        const int gr = r0 + rr, gk = k2c * GKC + kk;
+        double x = 0.0;
+        if (gr < rext && gk < K2)
+            x = op.base[base + (int64_t)gr * op.sr + (int64_t)k1 * op.s1 + (int64_t)gk * op.s2];
+        lds[kk * GPITCH + rr] = x;
+    }
+}
+
+__global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
+    __shared__ double As[GKC * GPITCH];
+    __shared__ double Bs[GKC * GPITCH];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = wave >> 1, wn = wave & 1;
+    const int ntn = (g.N + GT - 1) / GT;
+    const int tm = blockIdx.x / ntn, tn = blockIdx.x - tm * ntn;
+    const int m0 = tm * GT, n0 = tn * GT;
+    const int beta = blockIdx.y, split = blockIdx.z;
+
+    // chunk range of this split
+    int64_t total = 0;
+    for (int s = 0; s < g.nseg; s++)
+        total += (int64_t)g.seg[s].K1 * ((g.seg[s].K2 + GKC - 1) / GKC);
+    const int64_t c_begin = total * split / g.splits, c_end = total * (split + 1) / g.splits;
+
+    d4 acc[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++)
+            acc[x][y] = d4{0.0, 0.0, 0.0, 0.0};
+
+    int64_t cbase = 0;
+    for (int s = 0; s < g.nseg; s++) {
+        const GemmSeg &sg = g.seg[s];
+        const int nk2 = (sg.K2 + GKC - 1) / GKC;
+        const int64_t nch = (int64_t)sg.K1 * nk2;
+        const int64_t lo = c_begin > cbase ? c_begin - cbase : 0;
+        const int64_t hi = c_end - cbase < nch ? c_end - cbase : nch;
+        const int64_t abase = (int64_t)beta * sg.A.sb, bbase = (int64_t)beta * sg.B.sb;
+        for (int64_t ch = lo; ch < hi; ch++) {
+            const int k1 = (int)(ch / nk2), k2c = (int)(ch - (int64_t)k1 * nk2);
+            gemm_load_tile(sg.A, abase, m0, g.M, k1, k2c, sg.K2, As, tid);
+            gemm_load_tile(sg.B, bbase, n0, g.N, k1, k2c, sg.K2, Bs, tid);
+            __syncthreads();
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int kk = t * 4 + (lane >> 4);
+                double af[2], bf[2];
+#pragma unroll
+                for (int x = 0; x < 2; x++)
+                    af[x] = As[kk * GPITCH + wm * 32 + x * 16 + (lane & 15)];
+#pragma unroll
+                for (int y = 0; y < 2; y++)
+                    bf[y] = Bs[kk * GPITCH + wn * 32 + y * 16 + (lane & 15)];
+#pragma unroll
+                for (int x = 0; x < 2; x++)
+#pragma unroll
+                    for (int y = 0; y < 2; y++)
+                        acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[x], bf[y], acc[x][y], 0, 0, 0);
+            }
+            __syncthreads();
+        }
+        cbase += nch;
+    }
+
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++) {
+            const int n = n0 + wn * 32 + y * 16 + (lane & 15);
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int m = m0 + wm * 32 + x * 16 + (lane >> 4) + 4 * r;
+                if (m < g.M && n < g.N) {
+                    if (g.splits == 1) {
+                        double *cp = g.C + (int64_t)beta * g.sCb + (int64_t)m * g.sCm + (int64_t)n * g.sCn;
+                        *cp += g.alpha * acc[x][y][r];
+                    } else {
+                        g.scratch[((int64_t)split * g.M + m) * g.N + n] = acc[x][y][r];
+                    }
+                }
+            }
+        }
+}
+
+__global__ __launch_bounds__(256) void gemm_reduce_kernel(GemmArgs g) {
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (int64_t)g.M * g.N)
+        return;
+    const int m = (int)(idx / g.N), n = (int)(idx - (int64_t)m * g.N);
+    double s = 0.0;
+    for (int p = 0; p < g.splits; p++)
+        s += g.scratch[(int64_t)p * g.M * g.N + idx];
+    g.C[(int64_t)m * g.sCm + (int64_t)n * g.sCn] += g.alpha * s;
+}
+
+hipError_t launch_gemm(const GemmArgs &g, int batch, hipStream_t s) {
+    const int ntiles = ((g.M + GT - 1) / GT) * ((g.N + GT - 1) / GT);
+    hipLaunchKernelGGL(gemm_kernel, dim3((unsigned)ntiles, (unsigned)batch, (unsigned)g.splits), dim3(256), 0, s, g);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || g.splits == 1)
+        return e;
+    const int64_t n = (int64_t)g.M * g.N;
+    hipLaunchKernelGGL(gemm_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------- V backward
+// V(a,b,c) = U0[a][b] t_k^c + U1[a][c] t_j^b + U2[b][c] t_i^a (U rows pitched vR, t1 rows vR):
+//   dU0[a][b] = sum_c G_V t_k^c,  dU1[a][c] = sum_b G_V t_j^b,  dU2[b][c] = sum_a G_V t_i^a
+//   dt_k^c = sum_ab G_V U0[a][b], dt_j^b = sum_ac G_V U1[a][c], dt_i^a = sum_bc G_V U2[b][c]
+__global__ __launch_bounds__(256) void vback_du_kernel(const double *GV, const double *t1, int i, int j, int k, int v,
+                                                        int vR, double *dU) {
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t vv = (int64_t)v * v;
+    if (idx >= 3 * vv)
+        return;
+    const int w = (int)(idx / vv);
+    const int64_t xy = idx - w * vv;
+    const int x = (int)(xy / v), y = (int)(xy - (int64_t)x * v);
+    const int64_t R2 = (int64_t)vR * vR;
+    double s = 0.0, c = 0.0;
+    if (w == 0) {  // dU0[x=a][y=b]
+        const double *g = GV + x * R2 + (int64_t)y * vR;
+        const double *t = t1 + (int64_t)k * vR;
+        for (int z = 0; z < v; z++)
+            nadd_a(s, c, g[z] * t[z]);
+    } else if (w == 1) {  // dU1[x=a][y=c]
+        const double *t = t1 + (int64_t)j * vR;
+        for (int z = 0; z < v; z++)
+            nadd_a(s, c, GV[x * R2 + (int64_t)z * vR + y] * t[z]);
+    } else {  // dU2[x=b][y=c]
+        const double *t = t1 + (int64_t)i * vR;
+        for (int z = 0; z < v; z++)
+            nadd_a(s, c, GV[z * R2 + (int64_t)x * vR + y] * t[z]);
+    }
+    dU[idx] = s + c;
+}
+
+__global__ __launch_bounds__(256) void vback_dt_kernel(const double *GV, const double *U, int v, int vR, double *dt) {
+    __shared__ double rs[256], rc[256];
+    const int w = blockIdx.x / v, x = blockIdx.x - w * v;
+    const int64_t R2 = (int64_t)vR * vR, vr = (int64_t)v * vR;
+    const int64_t n = (int64_t)v * v;
+    double s = 0.0, c = 0.0;
+    for (int64_t t = threadIdx.x; t < n; t += 256) {
+        const int p = (int)(t / v), q = (int)(t - (int64_t)p * v);
+        double g, u;
+        if (w == 0) {  // dt_k[x=c] = sum_ab G[a][b][c] U0[a][b]
+            g = GV[p * R2 + (int64_t)q * vR + x];
+            u = U[(int64_t)p * vR + q];
+        } else if (w == 1) {  // dt_j[x=b] = sum_ac G[a][b][c] U1[a][c]
+            g = GV[p * R2 + (int64_t)x * vR + q];
+            u = U[vr + (int64_t)p * vR + q];
+        } else {  // dt_i[x=a] = sum_bc G[a][b][c] U2[b][c]
+            g = GV[x * R2 + (int64_t)p * vR + q];
+            u = U[2 * vr + (int64_t)p * vR + q];
+        }
+        nadd_a(s, c, g * u);
+    }
+    rs[threadIdx.x] = s;
+    rc[threadIdx.x] = c;
+    __syncthreads();
+    for (int m = 128; m > 0; m >>= 1) {
+        if ((int)threadIdx.x < m) {
+            const double a1 = rs[threadIdx.x], b1 = rs[threadIdx.x + m];
+            const double ss = a1 + b1, bp = ss - a1;
+            rc[threadIdx.x] = rc[threadIdx.x] + rc[threadIdx.x + m] + ((a1 - (ss - bp)) + (b1 - bp));
+            rs[threadIdx.x] = ss;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        dt[blockIdx.x] = rs[0] + rc[0];
+}
+
+// dst[x][y] += alpha * src[trans ? y : x][trans ? x : y] over v x v (dst pitch ldd, src pitch v)
+__global__ __launch_bounds__(256) void axpy2_kernel(double *dst, int64_t ldd, const double *src, int v, int trans,
+                                                     double alpha) {
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (int64_t)v * v)
+        return;
+    const int x = (int)(idx / v), y = (int)(idx - (int64_t)x * v);
+    dst[(int64_t)x * ldd + y] += alpha * (trans ? src[(int64_t)y * v + x] : src[idx]);
+}
+
+__global__ __launch_bounds__(256) void vadd_kernel(double *dst, const double *src, int n) {
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx < n)
+        dst[idx] += src[idx];
+}
+
+hipError_t launch_vback(const double *GV, const double *U, const double *t1, int i, int j, int k, int o, int v, int vR,
+                        double *dU, double *dt, double *dS, double *dt1, hipStream_t s) {
+    const int64_t vv = (int64_t)v * v;
+    hipLaunchKernelGGL(vback_du_kernel, dim3((unsigned)((3 * vv + 255) / 256)), dim3(256), 0, s, GV, t1, i, j, k, v, vR,
+                       dU);
+    hipLaunchKernelGGL(vback_dt_kernel, dim3((unsigned)(3 * v)), dim3(256), 0, s, GV, U, v, vR, dt);
+    // dS[p][q][x][y] = d/d(px|qy); six ordered updates, each its own launch (targets may coincide)
+    const dim3 gb((unsigned)((vv + 255) / 256));
+    const int pq[6][2] = {{i, j}, {j, i}, {i, k}, {k, i}, {j, k}, {k, j}};
+    for (int u = 0; u < 6; u++) {
+        double *dst = dS + ((int64_t)pq[u][0] * o + pq[u][1]) * vv;
+        hipLaunchKernelGGL(axpy2_kernel, gb, dim3(256), 0, s, dst, (int64_t)v, dU + (u / 2) * vv, v, u & 1, 0.5);
+    }
+    // dt1 rows (pitch vR): k from dt[0..v), j from dt[v..2v), i from dt[2v..3v); rows may coincide
+    const int rows[3] = {k, j, i};
+    for (int u = 0; u < 3; u++)
+        hipLaunchKernelGGL(vadd_kernel, dim3((unsigned)((v + 255) / 256)), dim3(256), 0, s, dt1 + (int64_t)rows[u] * vR,
+                           dt + (int64_t)u * v, v);
+    return hipGetLastError();
+}
+
+// --------------------------------------------------------------- unpacking
+// ColMajor host-layout gradients from the packed FP64 accumulators.
+__global__ __launch_bounds__(256) void unpack_kernel(int mode, const double *A, const double *B, double *out, int o_,
+                                                     int v_, int64_t vK, int64_t oK, int64_t vR) {
+    const int64_t o = o_, v = v_;
+    const int64_t gid = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    int64_t n = 0;
+    switch (mode) {
+    case 0: n = o * v * v * v; break;  // ovvv(p,x,y,f) = dX[p][x][y][f]
+    case 1: n = o * v * o * o; break;  // ovoo(p,x,m,q) = -dRn[p][q][x][m]
+    case 2: n = o * v * o * v; break;  // ovov(p,x,q,y) = dS[p][q][x][y]
+    case 3: n = o * o * v * v; break;  // t2(q,r,z,f) = dT[q][r][z][f] + dTM[r][z][f][q]
+    case 4: n = o * v; break;          // t1(k,c) = dt1[k][c]
+    }
+    if (gid >= n)
+        return;
+    double val = 0.0;
+    switch (mode) {
+    case 0: {
+        const int64_t p = gid % o, t = gid / o, x = t % v, t2 = t / v, y = t2 % v, f = t2 / v;
+        val = A[((p * v + x) * v + y) * vK + f];
+    } break;
+    case 1: {
+        const int64_t p = gid % o, t = gid / o, x = t % v, t2 = t / v, m = t2 % o, q = t2 / o;
+        val = -A[((p * o + q) * v + x) * oK + m];
+    } break;
+    case 2: {
+        const int64_t p = gid % o, t = gid / o, x = t % v, t2 = t / v, q = t2 % o, y = t2 / o;
+        val = A[((p * o + q) * v + x) * v + y];
+    } break;
+    case 3: {
+        const int64_t q = gid % o, t = gid / o, r = t % o, t2 = t / o, z = t2 % v, f = t2 / v;
+        val = A[((q * o + r) * v + z) * vK + f] + B[((r * v + z) * v + f) * oK + q];
+    } break;
+    case 4: {
+        const int64_t kk = gid % o, c = gid / o;
+        val = A[kk * vR + c];
+    } break;
+    }
+    out[gid] = val;
+}
+
+hipError_t launch_unpack(int mode, const double *A, const double *B, double *out, int o, int v, int64_t vK, int64_t oK,
+                         int64_t vR, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(unpack_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, mode, A, B, out, o, v, vK, oK,
+                       vR);
+    return hipGetLastError();
+}
+
+}  // namespace scess

@@ -44,7 +44,7 @@ struct WBuildArgs {
     int nt;            // tiles per row / column dimension
     int ntile1;        // work-groups of the fixed-a family
     const int *ijk;    // 3 ints per batch entry
-    void *W1, *W2;     // accumulator type, [nbatch][v][vR][vR]
+    void *W1, *W2;     // accumulator type, [nbatch] x vR^3 in 8x8x8 blocks
     int64_t wstride;   // elements per batch entry
 };
 
@@ -53,23 +53,60 @@ int wbuild_tile(int prec, int v);  // returns blk for this precision and size
 hipError_t launch_wbuild(int prec, int blk, const WBuildArgs &a, int nbatch, hipStream_t s);
 
 struct EpiArgs {
-    const void *W1, *W2;  // accumulator type, [nbatch][v][vR][vR]
+    const void *W1, *W2;  // accumulator type, [nbatch] x vR^3 in 8x8x8 blocks
     int64_t wstride;
     int o, v, vR, nb;     // nb = vR / 8 blocks per dimension
     const int *ijk;       // 3 ints per batch entry
     const int *orbits;    // 3 ints per orbit (A >= B >= C block indices)
     int norb;
-    const double *S;      // [o][o][v][v] S[p][q][x][y] = (px|qy)
-    const double *t1;     // [o][v]
+    const double *U;      // [nbatch][3][v][vR] V pieces (launch_vpieces), rows zero-padded
+    const double *t1;     // [o][vR], rows zero-padded
     const double *e;      // [o+v]
-    double *partials;     // [nbatch][norb]
+    const double *ev;     // [vR] virtual orbital energies, zero-padded
+    int ngrp;             // work-groups per ijk (runs of consecutive orbits)
+    double *partials;     // [nbatch][ngrp]
 };
+int epilogue_groups(int norb);
 hipError_t launch_epilogue(int prec, const EpiArgs &a, int nbatch, hipStream_t s);
+// V pieces of every batch entry from S[p][q][x][y] = (px|qy) into U[nbatch][3][v][vR]
+hipError_t launch_vpieces(const double *S, const int *ijk, int o, int v, int vR, double *U, int nbatch, hipStream_t s);
 
 // F[ijk_index[b]] = deterministic compensated sum of partials[b][0..norb)
 hipError_t launch_reduce_partials(const double *partials, int norb, const int *ijk_index, double *F, int nbatch,
                                   hipStream_t s);
 // out[0] = sum_t 2/deg_t F[t] over the unique occupied triples (deterministic, compensated)
 hipError_t launch_reduce_energy(const double *F, const int *ijk_all, int nall, double *out, hipStream_t s);
+
+// ----------------------------------------------------------------- adjoint
+// FP64 operand of the generic GEMM: element (r, k1, k2) of batch b at
+// base + b*sb + r*sr + k1*s1 + k2*s2.
+struct GemmOp {
+    const double *base;
+    int64_t sr, s1, s2, sb;
+};
+struct GemmSeg {
+    GemmOp A, B;  // A rows m, B rows n
+    int K1, K2;   // K = K1 x K2, K2 the inner (chunked) extent
+};
+// C[b][m][n] += alpha sum_seg sum_k A(m,k) B(n,k); splits > 1 (batch 1) sums K-range partials in order.
+struct GemmArgs {
+    GemmSeg seg[2];
+    int nseg;
+    int M, N;
+    double *C;
+    int64_t sCm, sCn, sCb;
+    double alpha;
+    int splits;
+    double *scratch;  // splits * M * N
+};
+hipError_t launch_gemm(const GemmArgs &g, int batch, hipStream_t s);
+// G_W, G_V ([vR]^3 row-major, FP64) of the single occupied triple a.ijk; W in a.W1/W2, V pieces in a.U
+hipError_t launch_adjoint_g(int prec, const EpiArgs &a, double wgt, double *GW, double *GV, hipStream_t s);
+// V backward of one occupied triple into dS [o][o][v][v] and dt1 [o][vR]; dU (3 v^2) and dt (3 v) are scratch
+hipError_t launch_vback(const double *GV, const double *U, const double *t1, int i, int j, int k, int o, int v, int vR,
+                        double *dU, double *dt, double *dS, double *dt1, hipStream_t s);
+// ColMajor gradients: mode 0 ovvv <- dX, 1 ovoo <- -dRn, 2 ovov <- dS, 3 t2 <- dT + dTM, 4 t1 <- dt1
+hipError_t launch_unpack(int mode, const double *A, const double *B, double *out, int o, int v, int64_t vK, int64_t oK,
+                         int64_t vR, int64_t n, hipStream_t s);
 
 }  // namespace scess

@@ -40,6 +40,15 @@ __device__ inline uint16_t bf16_rne(float f) {
 }
 __device__ inline float bf16_to_float(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
 
+// Work-group id -> logical id so that consecutive logical ids run on the same
+// XCD: the dispatcher hands work-groups to the 8 XCDs of the MI355X round-robin,
+// and neighbouring work items share operand rows and cache lines of W.
+__device__ inline int xcd_remap(int w, int n) {
+    const int xcd = w & 7, idx = w >> 3;
+    const int per = n >> 3, rem = n & 7;
+    return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+}
+
 // Neumaier compensated accumulation
 __device__ inline void nadd(double &s, double &c, double x) {
     const double t = s + x;
@@ -243,14 +252,17 @@ __global__ __launch_bounds__(256) void wbuild_kernel(WBuildArgs a) {
     const int lr = lane & 15, lq = lane >> 4;
 
     const int nt2 = a.nt * a.nt;
-    int bid = blockIdx.x;
+    const int gx = gridDim.x;
+    const int lin = xcd_remap(blockIdx.y * gx + blockIdx.x, gx * gridDim.y);
+    const int bat = lin / gx;
+    int bid = lin - bat * gx;
     const bool fam1 = bid < a.ntile1;
     if (!fam1)
         bid -= a.ntile1;
     const int64_t fx = bid / nt2;
     const int tt = bid - (int)fx * nt2;
     const int tr = tt / a.nt, tc = tt - tr * a.nt;
-    const int *q = a.ijk + 3 * blockIdx.y;
+    const int *q = a.ijk + 3 * bat;
     const int64_t i = q[0], j = q[1], k = q[2];
     const int v = a.v;
     const int r0 = tr * BM, c0 = tc * BM;
@@ -275,9 +287,10 @@ __global__ __launch_bounds__(256) void wbuild_kernel(WBuildArgs a) {
 
     int s = 0, kc = 0;
     Seg sg = get_seg(a, fam1, 0, i, j, k, fx);
-    uint4 ra[NVT], rb[NVT];
+    // two register stages: the global loads of chunk c+2 are issued before chunk c is computed
+    uint4 ra0[NVT], rb0[NVT], ra1[NVT], rb1[NVT];
 
-    auto load = [&]() {
+    auto load = [&](uint4 *ra, uint4 *rb) {
 #pragma unroll
         for (int u = 0; u < NVT; u++) {
             const int idx = tid + 256 * u;
@@ -295,7 +308,7 @@ __global__ __launch_bounds__(256) void wbuild_kernel(WBuildArgs a) {
                 sg = get_seg(a, fam1, s, i, j, k, fx);
         }
     };
-    auto store = [&](int buf) {
+    auto store = [&](int buf, const uint4 *ra, const uint4 *rb) {
         char *As = lds[buf];
         char *Bs = As + BM * PITCH;
 #pragma unroll
@@ -372,20 +385,32 @@ __global__ __launch_bounds__(256) void wbuild_kernel(WBuildArgs a) {
         }
     };
 
-    load();
-    store(0);
+    load(ra0, rb0);
+    store(0, ra0, rb0);
+    if (total > 1)
+        load(ra1, rb1);
     __syncthreads();
-    for (int c = 0; c < total; ++c) {
-        const bool more = c + 1 < total;
-        if (more)
-            load();
-        compute(c & 1);
-        if (more)
-            store((c + 1) & 1);
+    for (int c = 0; c < total; c += 2) {
+        if (c + 2 < total)
+            load(ra0, rb0);
+        compute(0);
+        if (c + 1 < total)
+            store(1, ra1, rb1);
+        __syncthreads();
+        if (c + 1 >= total)
+            break;
+        if (c + 3 < total)
+            load(ra1, rb1);
+        compute(1);
+        if (c + 2 < total)
+            store(0, ra0, rb0);
         __syncthreads();
     }
 
-    Acc *W = (Acc *)(fam1 ? a.W1 : a.W2) + (int64_t)blockIdx.y * a.wstride + fx * (int64_t)a.vR * a.vR;
+    // W in 8x8x8 blocks: element (fixed, row, col) at wblk(fixed, row, col)
+    Acc *W = (Acc *)(fam1 ? a.W1 : a.W2) + (int64_t)bat * a.wstride;
+    const int nbR = a.vR >> 3;
+    const int64_t fbase = ((int64_t)(fx >> 3) * nbR) * nbR * 512 + (fx & 7) * 64;
 #pragma unroll
     for (int x = 0; x < BLK; x++)
 #pragma unroll
@@ -395,7 +420,7 @@ __global__ __launch_bounds__(256) void wbuild_kernel(WBuildArgs a) {
             for (int r = 0; r < 4; r++) {
                 const int row = r0 + wm * BLK * 16 + x * 16 + (PREC == P64 ? lq + 4 * r : 4 * lq + r);
                 if (row < v && col < v)
-                    W[(int64_t)row * a.vR + col] = acc[x][y][r];
+                    W[fbase + ((int64_t)(row >> 3) * nbR + (col >> 3)) * 512 + (row & 7) * 8 + (col & 7)] = acc[x][y][r];
             }
         }
 }
@@ -415,6 +440,7 @@ static hipError_t launch_wbuild_p(int blk, const WBuildArgs &a, int nbatch, hipS
     switch (blk) {
     case 1: hipLaunchKernelGGL((wbuild_kernel<PREC, 1>), grid, dim3(256), 0, s, a); break;
     case 2: hipLaunchKernelGGL((wbuild_kernel<PREC, 2>), grid, dim3(256), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((wbuild_kernel<PREC, 4>), grid, dim3(256), 0, s, a); break;
     case 5: hipLaunchKernelGGL((wbuild_kernel<PREC, 5>), grid, dim3(256), 0, s, a); break;
     default: return hipErrorInvalidValue;
     }
@@ -430,134 +456,266 @@ hipError_t launch_wbuild(int prec, int blk, const WBuildArgs &a, int nbatch, hip
 }
 
 // ---------------------------------------------------------------- epilogue
-template <typename AccT>
-__global__ __launch_bounds__(256) void epilogue_kernel(EpiArgs a) {
+// 1/D to full FP64 accuracy: hardware reciprocal and two Newton steps
+__device__ inline double recip(double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-d, r, 1.0);
+    return fma(r, e, r);
+}
+
+constexpr int EPI_THREADS = 384;  // one thread per (block, a', b') of an orbit's six 8^3 blocks
+
+// U[b][0] = 1/2 (S_ij + S_ji^T), U[b][1] = 1/2 (S_ik + S_ki^T), U[b][2] = 1/2 (S_jk + S_kj^T)
+// (v x vR each, rows zero-padded to vR) for every batch entry b, so that the six GetV terms of Symmetrise read
+// V(x,y,z) = U0[x][y] t_k^z + U1[x][z] t_j^y + U2[y][z] t_i^x.
+__global__ __launch_bounds__(256) void vpieces_kernel(const double *__restrict__ S, const int *__restrict__ ijk, int o,
+                                                       int v, int vR, double *__restrict__ U) {
+    const int64_t vv = (int64_t)v * v, vr = (int64_t)v * vR;
+    const int64_t gid = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (gid >= 3 * vr)
+        return;
+    const int b = blockIdx.y;
+    const int *q = ijk + 3 * b;
+    const int w = (int)(gid / vr);
+    const int64_t xy = gid - w * vr;
+    const int x = (int)(xy / vR), y = (int)(xy - (int64_t)x * vR);
+    if (y >= v) {  // zero padding of each row to vR
+        U[(int64_t)b * 3 * vr + gid] = 0.0;
+        return;
+    }
+    int p1, p2;
+    if (w == 0) {
+        p1 = q[0];
+        p2 = q[1];
+    } else if (w == 1) {
+        p1 = q[0];
+        p2 = q[2];
+    } else {
+        p1 = q[1];
+        p2 = q[2];
+    }
+    U[(int64_t)b * 3 * vr + gid] =
+        0.5 * (S[((int64_t)p1 * o + p2) * vv + (int64_t)x * v + y] + S[((int64_t)p2 * o + p1) * vv + (int64_t)y * v + x]);
+}
+
+hipError_t launch_vpieces(const double *S, const int *ijk, int o, int v, int vR, double *U, int nbatch, hipStream_t s) {
+    const int64_t n = 3 * (int64_t)v * vR;
+    hipLaunchKernelGGL(vpieces_kernel, dim3((unsigned)((n + 255) / 256), (unsigned)nbatch), dim3(256), 0, s, S, ijk,
+                       o, v, vR, U);
+    return hipGetLastError();
+}
+
+// Deterministic (s, c) reduction within a wave: fixed xor-butterfly of TwoSum steps.
+__device__ inline void wave_reduce(double &s, double &c) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        const double so = __shfl_xor(s, m, 64), co = __shfl_xor(c, m, 64);
+        double ss, ee;
+        // order the operands by lane so both partners compute the same bits
+        const bool lo = (threadIdx.x & m) == 0;
+        two_sum(lo ? s : so, lo ? so : s, ss, ee);
+        c = (lo ? c + co : co + c) + ee;
+        s = ss;
+    }
+}
+
+// Geometry of one orbit {(A,B,C) and its permutations} of 8-wide virtual blocks
+// as seen by one thread owning row (d, a', b') of the distinct block d.
+struct OrbitRow {
+    bool own;       // d < number of distinct blocks
+    int ga, gb, gc0;  // global a, b and first c of the row
+    int nd;
+    int off[5];     // Yl offsets (block, x, y) of the five permuted reads at c' = 0
+    int step[5];    // Yl offset increment per c'
+};
+
+template <int PB, int PX, int PY>
+__device__ inline OrbitRow orbit_row(const int *orbits, int orb, int tid) {
     constexpr int EB = 8;
-    constexpr int LM = 3 * EB;  // largest local extent
-    __shared__ double Wl[6 * EB * EB * EB];
-    __shared__ double U[3][LM * LM];
-    __shared__ double tl[3][LM];
-    __shared__ double evl[LM];
-    __shared__ double red_s[256], red_c[256];
-    __shared__ int dist[6][3];
-    __shared__ int smap[27];
-    __shared__ int bl[3];
-    __shared__ int nd_s, nl_s;
+    OrbitRow r;
+    const int A = orbits[3 * orb], B = orbits[3 * orb + 1], C = orbits[3 * orb + 2];
+    const int bl1 = B != A ? B : C;
+    auto slot = [&](int blk) { return blk == A ? 0 : (blk == bl1 ? 1 : 2); };
+    const int sB = slot(B), sC = slot(C);
+    uint64_t map_lo = 0, map_hi = 0;
+    uint32_t seen = 0;
+    int nd = 0;
+    const int d = tid >> 6, ap = (tid >> 3) & 7, bp = tid & 7;
+    int P = 0, Q = 0, R = 0, sP = 0, sQ = 0, sR = 0;
+    auto add = [&](int p0, int p1, int p2, int s0, int s1, int s2) {
+        const int key = s0 * 9 + s1 * 3 + s2;
+        if (!(seen >> key & 1u)) {
+            seen |= 1u << key;
+            if (key < 21)
+                map_lo |= (uint64_t)nd << (3 * key);
+            else
+                map_hi |= (uint64_t)nd << (3 * (key - 21));
+            if (nd == d) {
+                P = p0;
+                Q = p1;
+                R = p2;
+                sP = s0;
+                sQ = s1;
+                sR = s2;
+            }
+            nd++;
+        }
+    };
+    add(A, B, C, 0, sB, sC);
+    add(B, C, A, sB, sC, 0);
+    add(C, A, B, sC, 0, sB);
+    add(C, B, A, sC, sB, 0);
+    add(A, C, B, 0, sC, sB);
+    add(B, A, C, sB, 0, sC);
+    auto dindex = [&](int s0, int s1, int s2) -> int {
+        const int key = s0 * 9 + s1 * 3 + s2;
+        return (int)((key < 21 ? map_lo >> (3 * key) : map_hi >> (3 * (key - 21))) & 7u);
+    };
+    r.nd = nd;
+    r.own = d < nd;
+    r.ga = P * EB + ap;
+    r.gb = Q * EB + bp;
+    r.gc0 = R * EB;
+    // Y(lb, lc, la), Y(lc, la, lb), Y(lc, lb, la), Y(la, lc, lb), Y(lb, la, lc) with lc = (sR, c')
+    r.off[0] = dindex(sQ, sR, sP) * PB + bp * PX + ap;
+    r.step[0] = PY;
+    r.off[1] = dindex(sR, sP, sQ) * PB + ap * PY + bp;
+    r.step[1] = PX;
+    r.off[2] = dindex(sR, sQ, sP) * PB + bp * PY + ap;
+    r.step[2] = PX;
+    r.off[3] = dindex(sP, sR, sQ) * PB + ap * PX + bp;
+    r.step[3] = PY;
+    r.off[4] = dindex(sQ, sP, sR) * PB + bp * PX + ap * PY;
+    r.step[4] = 1;
+    return r;
+}
+
+// One work-group per (ijk, run of EPI_RUN consecutive orbits).
+constexpr int EPI_RUN = 8;
+
+template <typename AccT>
+__global__ __launch_bounds__(EPI_THREADS) void epilogue_kernel(EpiArgs a) {
+    constexpr int EB = 8;
+    // Y of the orbit's distinct blocks, padded (pitches 9 and 73) against bank conflicts
+    constexpr int PY = 9, PX = EB * PY + 1, PB = EB * PX;
+    __shared__ double Yl[6 * PB];
+    __shared__ double red[2 * (EPI_THREADS / 64)];
 
     const int tid = threadIdx.x;
-    const int bat = blockIdx.y;
-    const int *ob = a.orbits + 3 * blockIdx.x;
+    const int nx = gridDim.x;
+    const int lin = xcd_remap(blockIdx.y * nx + blockIdx.x, nx * gridDim.y);
+    const int grp = lin % nx, bat = lin / nx;
     const int *q = a.ijk + 3 * bat;
     const int i = q[0], j = q[1], k = q[2];
     const int o = a.o, v = a.v;
-    const int64_t vR = a.vR;
+    const int vR = a.vR, nbR = a.vR >> 3;
+    const int64_t vr = (int64_t)v * vR;
+    const double eocc = a.e[i] + a.e[j] + a.e[k];
+    const double *U = a.U + (int64_t)bat * 3 * vr;
+    const AccT *W1b = (const AccT *)a.W1 + (int64_t)bat * a.wstride;
+    const AccT *W2b = (const AccT *)a.W2 + (int64_t)bat * a.wstride;
+    const int ap = (tid >> 3) & 7;
+    const int o0 = grp * EPI_RUN, o1 = min(o0 + EPI_RUN, a.norb);
 
-    if (tid == 0) {
-        const int A = ob[0], B = ob[1], C = ob[2];
-        int nl = 0;
-        bl[nl++] = A;
-        if (B != A)
-            bl[nl++] = B;
-        if (C != B)
-            bl[nl++] = C;
-        for (int t = nl; t < 3; t++)
-            bl[t] = -1;
-        for (int t = 0; t < 27; t++)
-            smap[t] = -1;
-        const int perm[6][3] = {{A, B, C}, {B, C, A}, {C, A, B}, {C, B, A}, {A, C, B}, {B, A, C}};
-        int nd = 0;
-        for (int p = 0; p < 6; p++) {
-            int key = 0;
-            for (int d = 0; d < 3; d++) {
-                const int sl = perm[p][d] == bl[0] ? 0 : (perm[p][d] == bl[1] ? 1 : 2);
-                key = key * 3 + sl;
+    // register stage: W (FP64), Y, virtual orbital energies of the row's 8 c'. Every row
+    // array is padded to vR, so the 8-wide rows load as whole vectors; columns past v are
+    // masked where they are used.
+    double Wn[EB], Yn[EB], En[EB];
+    OrbitRow rn;
+    auto fetch = [&](int orb) {
+        rn = orbit_row<PB, PX, PY>(a.orbits, orb, tid);
+        if (rn.own && rn.ga < v && rn.gb < v) {
+            // W1 block (P,Q,R) row (a', b') and W2 block (Q,P,R) row (b', a'), 8 contiguous c'
+            const int P = rn.ga >> 3, Q = rn.gb >> 3, R = rn.gc0 >> 3;
+            const AccT *W1 = W1b + (((int64_t)P * nbR + Q) * nbR + R) * 512 + (rn.ga & 7) * 64 + (rn.gb & 7) * 8;
+            const AccT *W2 = W2b + (((int64_t)Q * nbR + P) * nbR + R) * 512 + (rn.gb & 7) * 64 + (rn.ga & 7) * 8;
+            const double u0 = U[(int64_t)rn.ga * vR + rn.gb];
+            const double *u1 = U + vr + (int64_t)rn.ga * vR + rn.gc0;
+            const double *u2 = U + 2 * vr + (int64_t)rn.gb * vR + rn.gc0;
+            const double *tk = a.t1 + (int64_t)k * vR + rn.gc0;
+            const double *evp = a.ev + rn.gc0;
+            const double ti = a.t1[(int64_t)i * vR + rn.ga], tj = a.t1[(int64_t)j * vR + rn.gb];
+            const double eab = eocc - a.ev[rn.ga] - a.ev[rn.gb];
+            AccT w1[EB], w2[EB];
+            double x1[EB], x2[EB], xk[EB], xe[EB];
+#pragma unroll
+            for (int cp = 0; cp < EB; cp++) {
+                w1[cp] = W1[cp];
+                w2[cp] = W2[cp];
+                x1[cp] = u1[cp];
+                x2[cp] = u2[cp];
+                xk[cp] = tk[cp];
+                xe[cp] = evp[cp];
             }
-            if (smap[key] < 0) {
-                smap[key] = nd;
-                dist[nd][0] = perm[p][0];
-                dist[nd][1] = perm[p][1];
-                dist[nd][2] = perm[p][2];
-                nd++;
+#pragma unroll
+            for (int cp = 0; cp < EB; cp++) {
+                const double w = (double)w1[cp] + (double)w2[cp];
+                Wn[cp] = w;
+                Yn[cp] = w + u0 * xk[cp] + x1[cp] * tj + x2[cp] * ti;
+                En[cp] = eab - xe[cp];
+            }
+        } else {
+#pragma unroll
+            for (int cp = 0; cp < EB; cp++) {
+                Wn[cp] = Yn[cp] = 0.0;
+                En[cp] = 1.0;
             }
         }
-        nd_s = nd;
-        nl_s = nl;
-    }
-    __syncthreads();
-    const int nd = nd_s, nl = nl_s, L = nl * EB;
-
-    // W of the distinct blocks of the orbit
-    const AccT *W1 = (const AccT *)a.W1 + (int64_t)bat * a.wstride;
-    const AccT *W2 = (const AccT *)a.W2 + (int64_t)bat * a.wstride;
-    for (int idx = tid; idx < nd * 512; idx += 256) {
-        const int d = idx >> 9, e = idx & 511;
-        const int ga = dist[d][0] * EB + (e >> 6), gb = dist[d][1] * EB + ((e >> 3) & 7),
-                  gc = dist[d][2] * EB + (e & 7);
-        double w = 0.0;
-        if (ga < v && gb < v && gc < v)
-            w = (double)W1[(ga * vR + gb) * vR + gc] + (double)W2[(gb * vR + ga) * vR + gc];
-        Wl[idx] = w;
-    }
-    // V pieces: V(x,y,z) = Uk[x][y] t_k[z] + Uj[x][z] t_j[y] + Ui[y][z] t_i[x]
-    const double *S = a.S;
-    const int64_t vv = (int64_t)v * v;
-    for (int idx = tid; idx < L * L; idx += 256) {
-        const int u1 = idx / L, u2 = idx - u1 * L;
-        const int x = bl[u1 >> 3] * EB + (u1 & 7), y = bl[u2 >> 3] * EB + (u2 & 7);
-        double uk = 0.0, uj = 0.0, ui = 0.0;
-        if (x < v && y < v) {
-            uk = 0.5 * (S[((int64_t)i * o + j) * vv + (int64_t)x * v + y] + S[((int64_t)j * o + i) * vv + (int64_t)y * v + x]);
-            uj = 0.5 * (S[((int64_t)i * o + k) * vv + (int64_t)x * v + y] + S[((int64_t)k * o + i) * vv + (int64_t)y * v + x]);
-            ui = 0.5 * (S[((int64_t)j * o + k) * vv + (int64_t)x * v + y] + S[((int64_t)k * o + j) * vv + (int64_t)y * v + x]);
-        }
-        U[0][u1 * LM + u2] = uk;
-        U[1][u1 * LM + u2] = uj;
-        U[2][u1 * LM + u2] = ui;
-    }
-    if (tid < L) {
-        const int g = bl[tid >> 3] * EB + (tid & 7);
-        const bool ok = g < v;
-        tl[0][tid] = ok ? a.t1[(int64_t)i * v + g] : 0.0;
-        tl[1][tid] = ok ? a.t1[(int64_t)j * v + g] : 0.0;
-        tl[2][tid] = ok ? a.t1[(int64_t)k * v + g] : 0.0;
-        evl[tid] = ok ? a.e[o + g] : 0.0;
-    }
-    __syncthreads();
-
-    auto Yat = [&](int x, int y, int z) -> double {
-        const int d = smap[(x >> 3) * 9 + (y >> 3) * 3 + (z >> 3)];
-        return Wl[(d << 9) + ((x & 7) << 6) + ((y & 7) << 3) + (z & 7)] + U[0][x * LM + y] * tl[2][z] +
-               U[1][x * LM + z] * tl[1][y] + U[2][y * LM + z] * tl[0][x];
     };
 
-    const double eocc = a.e[i] + a.e[j] + a.e[k];
     double s = 0.0, c = 0.0;
-    for (int idx = tid; idx < nd * 512; idx += 256) {
-        const int d = idx >> 9, e = idx & 511;
-        const int P = dist[d][0], Q = dist[d][1], R = dist[d][2];
-        const int ap = e >> 6, bp = (e >> 3) & 7, cp = e & 7;
-        if (P * EB + ap >= v || Q * EB + bp >= v || R * EB + cp >= v)
-            continue;
-        const int sP = P == bl[0] ? 0 : (P == bl[1] ? 1 : 2);
-        const int sQ = Q == bl[0] ? 0 : (Q == bl[1] ? 1 : 2);
-        const int sR = R == bl[0] ? 0 : (R == bl[1] ? 1 : 2);
-        const int la = sP * EB + ap, lb = sQ * EB + bp, lc = sR * EB + cp;
-        const double r3 = 4.0 * Yat(la, lb, lc) + Yat(lb, lc, la) + Yat(lc, la, lb) -
-                          2.0 * (Yat(lc, lb, la) + Yat(la, lc, lb) + Yat(lb, la, lc));
-        const double D = eocc - evl[la] - evl[lb] - evl[lc];
-        nadd(s, c, Wl[idx] * r3 / D);
+    for (int orb = o0; orb < o1; orb++) {
+        fetch(orb);
+        const OrbitRow &r = rn;
+        const int d = tid >> 6, bp = tid & 7;
+        if (r.own)
+#pragma unroll
+            for (int cp = 0; cp < EB; cp++)
+                Yl[d * PB + ap * PX + bp * PY + cp] = Yn[cp];
+        __syncthreads();
+        if (r.own && r.ga < v && r.gb < v) {
+#pragma unroll
+            for (int cp = 0; cp < EB; cp++) {
+                if (r.gc0 + cp < v) {  // Yl of masked columns is finite: padding is zero
+                    const double r3 = 4.0 * Yn[cp] + Yl[r.off[0] + cp * r.step[0]] + Yl[r.off[1] + cp * r.step[1]] -
+                                      2.0 * (Yl[r.off[2] + cp * r.step[2]] + Yl[r.off[3] + cp * r.step[3]] +
+                                             Yl[r.off[4] + cp * r.step[4]]);
+                    nadd(s, c, Wn[cp] * r3 * recip(En[cp]));
+                }
+            }
+        }
+        __syncthreads();
     }
-    const double tot = block_reduce_256(s, c, red_s, red_c);
-    if (tid == 0)
-        a.partials[(int64_t)bat * a.norb + blockIdx.x] = tot;
+    // fixed-order reduction: xor butterfly in each wave, then the waves in order
+    wave_reduce(s, c);
+    const int wv = tid >> 6;
+    if ((tid & 63) == 0) {
+        red[2 * wv] = s;
+        red[2 * wv + 1] = c;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double ts = red[0], tc = red[1];
+        for (int w = 1; w < EPI_THREADS / 64; w++) {
+            double ss, ee;
+            two_sum(ts, red[2 * w], ss, ee);
+            tc = tc + red[2 * w + 1] + ee;
+            ts = ss;
+        }
+        a.partials[(int64_t)bat * a.ngrp + grp] = ts + tc;
+    }
 }
 
+int epilogue_groups(int norb) { return (norb + EPI_RUN - 1) / EPI_RUN; }
+
 hipError_t launch_epilogue(int prec, const EpiArgs &a, int nbatch, hipStream_t s) {
-    const dim3 grid((unsigned)a.norb, (unsigned)nbatch);
+    const dim3 grid((unsigned)a.ngrp, (unsigned)nbatch);
     if (prec == P64)
-        hipLaunchKernelGGL(epilogue_kernel<double>, grid, dim3(256), 0, s, a);
+        hipLaunchKernelGGL(epilogue_kernel<double>, grid, dim3(EPI_THREADS), 0, s, a);
     else
-        hipLaunchKernelGGL(epilogue_kernel<float>, grid, dim3(256), 0, s, a);
+        hipLaunchKernelGGL(epilogue_kernel<float>, grid, dim3(EPI_THREADS), 0, s, a);
     return hipGetLastError();
 }
 

@@ -93,8 +93,8 @@ def load(path: str = LIB_PATH):
     lib.scess_t_forward_ex.restype = c.c_int
     lib.scess_t_num_ijk.argtypes = [c.c_void_p]
     lib.scess_t_num_ijk.restype = c.c_int64
-    lib.scess_t_num_orbits.argtypes = [c.c_void_p]
-    lib.scess_t_num_orbits.restype = c.c_int64
+    lib.scess_t_num_partials.argtypes = [c.c_void_p]
+    lib.scess_t_num_partials.restype = c.c_int64
     if hasattr(lib, "scess_t_backward"):
         lib.scess_t_backward.argtypes = [c.c_void_p, c.c_int, c.c_double, c.POINTER(TGrad)]
         lib.scess_t_backward.restype = c.c_int
@@ -183,7 +183,7 @@ class Context:
         if with_F:
             F = np.zeros(self.lib.scess_t_num_ijk(self.h))
         if with_partials:
-            P = np.zeros((self.lib.scess_t_num_ijk(self.h), self.lib.scess_t_num_orbits(self.h)))
+            P = np.zeros((self.lib.scess_t_num_ijk(self.h), self.lib.scess_t_num_partials(self.h)))
         rc = self.lib.scess_t_forward_ex(self.h, PRECISIONS[precision], ctypes.byref(E),
                                          _ptr(F) if F is not None else None,
                                          _ptr(P) if P is not None else None, ctypes.byref(st))

@@ -13,7 +13,7 @@ from scess_amd import gpu
 def test_header_declares_expected_entry_points():
     names = set(gpu.header_functions())
     for n in ("scess_create", "scess_destroy", "scess_last_error", "scess_t_set_inputs", "scess_t_forward",
-              "scess_t_forward_ex", "scess_t_num_ijk", "scess_t_num_orbits", "scess_set_should_stop"):
+              "scess_t_forward_ex", "scess_t_num_ijk", "scess_t_num_partials", "scess_set_should_stop"):
         assert n in names
 
 

@@ -1,0 +1,58 @@
+"""Pins the CCSD oracle (oracle/ccsd_oracle.py, restating src/Quantum/CC/RCCSD.cpp)
+against an independent spin-orbital CCSD written from the textbook equations,
+plus structural invariants of the closed-shell update."""
+
+import numpy as np
+import pytest
+
+from oracle import ccsd_oracle as co
+from scess_amd.synthetic import _draw, full_eri_chem
+from spin_orbital_ccsd import ccsd_energy, spin_integrals
+
+
+def _system(o, v, seed, scale):
+    e, L, _ = _draw(o, v, seed)
+    return full_eri_chem(L * scale), e
+
+
+@pytest.mark.parametrize("o,v,seed,scale", [(2, 3, 1, 0.3), (2, 4, 2, 0.3), (3, 4, 3, 0.25)])
+def test_closed_shell_equals_spin_orbital(o, v, seed, scale):
+    eri, e = _system(o, v, seed, scale)
+    r = co.solve_rccsd_diis(eri, e, o)
+    assert r["converged"]
+    g, es = spin_integrals(eri, e, o)
+    Es = ccsd_energy(g, es, 2 * o)
+    assert abs(r["E_CCSD"] - Es) < 1e-9
+
+
+def test_update_preserves_pair_symmetry():
+    eri, e = _system(3, 5, 4, 0.3)
+    ints = co.make_blocks(eri, 3)
+    eia, eijab = co.denominators(e, 3)
+    rng = np.random.default_rng(0)
+    t1 = rng.standard_normal((3, 5)) * 1e-2
+    t2 = rng.standard_normal((3, 3, 5, 5)) * 1e-2
+    t2 = t2 + t2.transpose(1, 0, 3, 2)
+    n1, n2 = co.update_amplitudes(ints, eia, eijab, None, t1, t2)
+    assert np.allclose(n2, n2.transpose(1, 0, 3, 2), rtol=0, atol=1e-15)
+
+
+def test_first_energy_is_mp2_guess():
+    eri, e = _system(2, 4, 5, 0.3)
+    ints = co.make_blocks(eri, 2)
+    eia, eijab = co.denominators(e, 2)
+    t2 = ints["oovv_phys"] / eijab
+    emp2 = co.correlation_energy(ints, np.zeros((2, 4)), t2)
+    ref = np.sum((2 * ints["oovv_phys"] - ints["oovv_phys"].transpose(0, 1, 3, 2)) * ints["oovv_phys"] / eijab)
+    assert abs(emp2 - ref) < 1e-15
+
+
+def test_f12_driver_and_coupling_energy():
+    eri, e = _system(2, 3, 6, 0.3)
+    rng = np.random.default_rng(1)
+    res = rng.standard_normal((2, 2, 3, 3)) * 1e-3
+    res = res + res.transpose(1, 0, 3, 2)
+    en = rng.standard_normal((2, 2, 3, 3)) * 1e-3
+    r = co.solve_rccsd_diis(eri, e, 2, f12={"residual": res, "energy": en})
+    assert r["converged"]
+    assert np.isfinite(r["E_CCSD"])
