@@ -124,6 +124,50 @@ int64_t scess_t_num_partials(const scess_ctx *ctx);
  * scess_t_set_inputs. Deterministic: triples are processed in a fixed order. */
 int scess_t_backward(scess_ctx *ctx, int prec, double dE, scess_t_grad *grad);
 
+/* -- closed-shell CCSD ---------------------------------------------------------
+ * Integral blocks of SolveRCCSD_DIIS (src/Quantum/CC/RCCSD.cpp:31-44), chemists'
+ * notation, ColMajor with the index order as named. Either vvvv or chol_vv must
+ * be given; with chol_vv the particle-particle ladder regenerates (ac|bd) tiles
+ * and no v^4 tensor is formed. */
+typedef struct scess_ccsd_in {
+    int64_t o, v;
+    const double *e;          /* [o+v] canonical orbital energies, occupied first */
+    const double *ovov;       /* (k,c,l,d) = (kc|ld) */
+    const double *oovv;       /* (k,i,a,c) = (ki|ac) */
+    const double *ovvo;       /* (k,c,a,i) = (kc|ai) */
+    const double *ovvv;       /* (k,d,a,c) = (kd|ac) */
+    const double *ovoo;       /* (l,c,k,i) = (lc|ki) */
+    const double *oooo;       /* (k,i,l,j) = (ki|lj) */
+    const double *vvvv;       /* (a,c,b,d) = (ac|bd), or NULL */
+    const double *chol_vv;    /* (a,c,P): (ac|bd) = sum_P L(a,c,P) L(b,d,P), or NULL */
+    int64_t naux;             /* P extent of chol_vv */
+    const double *t2_driver;  /* optional (i,j,a,b): RCCSD_F12Coupling::residual (CC.h:121-124) */
+    const double *f12_energy; /* optional (i,j,a,b): RCCSD_F12Coupling::energy */
+} scess_ccsd_in;
+
+typedef struct scess_ccsd_result {
+    double E_CCSD;            /* correlation energy (coupling included), as RCCSD_Wfn::E_CCSD */
+    double E_F12_Coupling;    /* RCCSD_Wfn::E_F12_Coupling */
+    int converged;
+    int iterations;
+    double last_dE, last_rms;
+} scess_ccsd_result;
+
+/* Per-iteration progress (the reference prints "CCSD(n): E(corr) dE rms(dt)", RCCSD.cpp:366-368). */
+typedef void (*scess_ccsd_log)(void *user, int iteration, double e_corr, double dE, double rms);
+
+int scess_ccsd_set_inputs(scess_ctx *ctx, const scess_ccsd_in *in);
+/* One Jacobi update t <- R(t)/D (UpdateAmplitudes, RCCSD.cpp:101-237); t1 (i,a), t2 (i,j,a,b) ColMajor. */
+int scess_ccsd_update(scess_ctx *ctx, const double *t1, const double *t2, double *t1_new, double *t2_new);
+/* E = sum (2<ij|ab> - <ij|ba>) tau_ij^ab (+ F12 coupling) (RCCSD.cpp:85-98). */
+int scess_ccsd_energy(scess_ctx *ctx, const double *t1, const double *t2, double *e_corr);
+/* SolveRCCSD_DIIS (RCCSD.cpp:295-394): MP2 guess, <= 100 iterations, DIIS (8 vectors) from iteration 2,
+ * converged when dE < 1e-10 and rms(dt) < 1e-8. Amplitudes, DIIS vectors and dot products stay on the
+ * GPU in FP64; the bordered DIIS system is solved on the host with full pivoting. t1_out / t2_out may
+ * be NULL. */
+int scess_ccsd_solve(scess_ctx *ctx, double *t1_out, double *t2_out, scess_ccsd_result *res, scess_ccsd_log log,
+                     void *user);
+
 #ifdef __cplusplus
 }
 #endif

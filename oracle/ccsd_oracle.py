@@ -146,6 +146,38 @@ def update_amplitudes(ints, eia, eijab, t2driver, t1, t2):
     return t1new / eia, t2new / eijab
 
 
+def solve_full_pivot(A, b):
+    """LU with full pivoting (Eigen's fullPivLu): zero pivots give zero components, so
+    rank-deficient but consistent systems (e.g. identical DIIS error vectors) still solve."""
+    A = np.array(A, dtype=float)
+    b = np.array(b, dtype=float)
+    n = A.shape[0]
+    cp = np.arange(n)
+    for k in range(n):
+        sub = np.abs(A[k:, k:])
+        if sub.max() == 0.0:
+            break
+        pr, pc = np.unravel_index(np.argmax(sub), sub.shape)
+        pr += k
+        pc += k
+        A[[k, pr]] = A[[pr, k]]
+        b[[k, pr]] = b[[pr, k]]
+        A[:, [k, pc]] = A[:, [pc, k]]
+        cp[[k, pc]] = cp[[pc, k]]
+        for i in range(k + 1, n):
+            f = A[i, k] / A[k, k]
+            A[i, k:] -= f * A[k, k:]
+            b[i] -= f * b[k]
+    y = np.zeros(n)
+    for k in range(n - 1, -1, -1):
+        if A[k, k] == 0.0:
+            continue
+        y[k] = (b[k] - A[k, k + 1:] @ y[k + 1:]) / A[k, k]
+    x = np.zeros(n)
+    x[cp] = y
+    return x
+
+
 class AmplitudeDIIS:
     """RCCSD.cpp:240-279."""
 
@@ -170,7 +202,7 @@ class AmplitudeDIIS:
             B[:n, :n] /= mx
         rhs = np.zeros(n + 1)
         rhs[n] = -1.0
-        coef = np.linalg.solve(B, rhs)
+        coef = solve_full_pivot(B, rhs)
         out = np.zeros_like(amplitudes)
         for i in range(n):
             out += coef[i] * self.amps[i]

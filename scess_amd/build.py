@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libscess_gpu.so")
-SOURCES = ["triples_kernels.hip", "triples_adjoint.hip", "scess_capi.hip"]
+SOURCES = ["triples_kernels.hip", "triples_adjoint.hip", "ccsd_engine.hip", "scess_capi.hip"]
 ARCH = "gfx950"
 
 # Packed FP32 instructions are disabled for all device code.
@@ -51,7 +51,7 @@ def build(verbose: bool = False) -> str:
             print(out)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    _run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    _run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,--no-undefined", "-o", tmp, *objs])
     os.replace(tmp, LIB)
     return LIB
 

@@ -1,0 +1,552 @@
+// Closed-shell CCSD residual engine on gfx950 (FP64).
+//
+// A device einsum engine with the semantics of the reference's compile-time
+// Einsum (src/Tensor/TensorEinsum.h:9-17,137-205): labels shared by both
+// operands and absent from the output are summed; labels of the output found
+// in both operands are batch indices. Each contraction is evaluated as
+// permute(A) -> [batch][M][K], permute(B) -> [batch][N][K], one FP64 MFMA GEMM
+// (launch_gemm of triples_adjoint.hip), and a permuting accumulate into the
+// output. On top of it, UpdateAmplitudes (src/Quantum/CC/RCCSD.cpp:101-237) is
+// transcribed term by term; the particle-particle ladder sum_cd Wvvvv tau never
+// forms a v^4 tensor when the Cholesky factor of (ac|bd) is given:
+//   sum_cd (ac|bd) tau_ij^cd   from (ac|bd) tiles regenerated from L per (a,b) tile,
+//   sum_cd (kd|ac) t_k^b tau_ij^cd and sum_cd (kc|bd) t_k^a tau_ij^cd
+//                              through Z[k][a][i][j] = sum_cd (kd|ac) tau_ij^cd.
+#include "ccsd_internal.h"
+
+#include <algorithm>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+namespace scess {
+
+// ------------------------------------------------------------------ kernels
+constexpr int MAXD = 8;
+struct PermArgs {
+    int nd;
+    int64_t odims[MAXD];   // output dims (row-major)
+    int64_t istride[MAXD]; // input stride of each output dim
+    const double *in;
+    double *out;
+    double alpha, beta;    // out = beta * out + alpha * in[perm]
+    int64_t n;
+};
+
+__global__ __launch_bounds__(256) void permute_kernel(PermArgs a) {
+    const int64_t lin = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (lin >= a.n)
+        return;
+    int64_t r = lin, off = 0;
+    for (int d = a.nd - 1; d >= 0; d--) {
+        const int64_t x = r % a.odims[d];
+        r /= a.odims[d];
+        off += x * a.istride[d];
+    }
+    const double x = a.alpha * a.in[off];
+    a.out[lin] = a.beta == 0.0 ? x : a.beta * a.out[lin] + x;
+}
+
+// out = alpha * x + beta * y (elementwise)
+__global__ __launch_bounds__(256) void axpby_kernel(double *out, const double *x, const double *y, double alpha,
+                                                    double beta, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n)
+        out[i] = alpha * x[i] + (y ? beta * y[i] : 0.0);
+}
+
+// tau[i][j][a][b] = t2 + t1[i][a] t1[j][b]
+__global__ __launch_bounds__(256) void tau_kernel(double *tau, const double *t2, const double *t1, int o, int v) {
+    const int64_t n = (int64_t)o * o * v * v;
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= n)
+        return;
+    const int b = (int)(idx % v);
+    int64_t r = idx / v;
+    const int a = (int)(r % v);
+    r /= v;
+    const int j = (int)(r % o), i = (int)(r / o);
+    tau[idx] = t2[idx] + t1[(int64_t)i * v + a] * t1[(int64_t)j * v + b];
+}
+
+// t1 /= (e_i - e_a), t2 /= (e_i + e_j - e_a - e_b)   (RCCSD.cpp:235-236, :321-331)
+__global__ __launch_bounds__(256) void jacobi_kernel(double *t1, double *t2, const double *e, int o, int v) {
+    const int64_t n2 = (int64_t)o * o * v * v, n1 = (int64_t)o * v;
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx < n2) {
+        const int b = (int)(idx % v);
+        int64_t r = idx / v;
+        const int a = (int)(r % v);
+        r /= v;
+        const int j = (int)(r % o), i = (int)(r / o);
+        t2[idx] /= (e[i] - e[o + a]) + (e[j] - e[o + b]);
+    }
+    if (idx < n1) {
+        const int a = (int)(idx % v), i = (int)(idx / v);
+        t1[idx] /= e[i] - e[o + a];
+    }
+}
+
+// partial dot products, fixed order: part[block] = sum over the block's range
+__global__ __launch_bounds__(256) void dot_kernel(const double *x, const double *y, int64_t n, double *part) {
+    __shared__ double rs[256], rc[256];
+    const int64_t per = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t b0 = (int64_t)blockIdx.x * per, b1 = min(n, b0 + per);
+    double s = 0.0, c = 0.0;
+    for (int64_t t = b0 + threadIdx.x; t < b1; t += 256) {
+        const double p = x[t] * y[t];
+        const double u = s + p;
+        if (fabs(s) >= fabs(p))
+            c += (s - u) + p;
+        else
+            c += (p - u) + s;
+        s = u;
+    }
+    rs[threadIdx.x] = s;
+    rc[threadIdx.x] = c;
+    __syncthreads();
+    for (int m = 128; m > 0; m >>= 1) {
+        if ((int)threadIdx.x < m) {
+            const double a1 = rs[threadIdx.x], b1v = rs[threadIdx.x + m];
+            const double ss = a1 + b1v, bp = ss - a1;
+            rc[threadIdx.x] = rc[threadIdx.x] + rc[threadIdx.x + m] + ((a1 - (ss - bp)) + (b1v - bp));
+            rs[threadIdx.x] = ss;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        part[blockIdx.x] = rs[0] + rc[0];
+}
+
+hipError_t ccsd_jacobi(double *t1, double *t2, const double *e, int o, int v, hipStream_t s) {
+    const int64_t n = (int64_t)o * o * v * v;
+    hipLaunchKernelGGL(jacobi_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, t1, t2, e, o, v);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------- engine
+DTensor::~DTensor() { release(); }
+
+void DTensor::release() {
+    if (p)
+        (void)hipFreeAsync(p, s);
+    p = nullptr;
+}
+
+DTensor::DTensor(DTensor &&o) noexcept : p(o.p), dims(std::move(o.dims)), s(o.s) { o.p = nullptr; }
+
+DTensor &DTensor::operator=(DTensor &&o) noexcept {
+    if (this != &o) {
+        release();
+        p = o.p;
+        dims = std::move(o.dims);
+        s = o.s;
+        o.p = nullptr;
+    }
+    return *this;
+}
+
+int64_t DTensor::size() const {
+    int64_t n = 1;
+    for (auto d : dims)
+        n *= d;
+    return n;
+}
+
+static void check(hipError_t e, const char *what) {
+    if (e != hipSuccess)
+        throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+DTensor Engine::alloc(std::vector<int64_t> dims, bool zero) {
+    DTensor t;
+    t.dims = std::move(dims);
+    t.s = stream;
+    const size_t bytes = sizeof(double) * (size_t)std::max<int64_t>(t.size(), 1);
+    check(hipMallocAsync((void **)&t.p, bytes, stream), "hipMallocAsync (CCSD tensor)");
+    if (zero)
+        check(hipMemsetAsync(t.p, 0, bytes, stream), "hipMemsetAsync");
+    return t;
+}
+
+void Engine::upload(DTensor &t, const double *host) {
+    check(hipMemcpyAsync(t.p, host, sizeof(double) * (size_t)t.size(), hipMemcpyHostToDevice, stream), "upload");
+}
+
+void Engine::download(const DTensor &t, double *host) {
+    check(hipMemcpyAsync(host, t.p, sizeof(double) * (size_t)t.size(), hipMemcpyDeviceToHost, stream), "download");
+    check(hipStreamSynchronize(stream), "download sync");
+}
+
+void Engine::copy(DTensor &dst, const DTensor &src) {
+    check(hipMemcpyAsync(dst.p, src.p, sizeof(double) * (size_t)src.size(), hipMemcpyDeviceToDevice, stream), "copy");
+}
+
+void Engine::axpby(double *out, const double *x, const double *y, double alpha, double beta, int64_t n) {
+    hipLaunchKernelGGL(axpby_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, out, x, y, alpha, beta, n);
+    check(hipGetLastError(), "axpby");
+}
+
+// out (labels lo, row-major dims) = beta*out + alpha*in[permuted], in has labels li
+void Engine::permute(const std::string &li, const double *in, const std::vector<int64_t> &idims,
+                     const std::string &lo, double *out, const std::vector<int64_t> &odims, double alpha,
+                     double beta) {
+    PermArgs a{};
+    a.nd = (int)lo.size();
+    if (a.nd > MAXD || (int)li.size() != a.nd)
+        throw std::runtime_error("permute: rank");
+    std::vector<int64_t> is(li.size());
+    int64_t s = 1;
+    for (int d = (int)li.size() - 1; d >= 0; d--) {
+        is[d] = s;
+        s *= idims[d];
+    }
+    a.n = 1;
+    for (int d = 0; d < a.nd; d++) {
+        const auto pos = li.find(lo[d]);
+        if (pos == std::string::npos)
+            throw std::runtime_error("permute: label " + std::string(1, lo[d]) + " missing");
+        a.odims[d] = odims[d];
+        a.istride[d] = is[pos];
+        a.n *= odims[d];
+    }
+    a.in = in;
+    a.out = out;
+    a.alpha = alpha;
+    a.beta = beta;
+    if (a.n == 0)
+        return;
+    hipLaunchKernelGGL(permute_kernel, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, stream, a);
+    check(hipGetLastError(), "permute");
+}
+
+static void parse_spec(const std::string &spec, std::string &la, std::string &lb, std::string &lc) {
+    const auto arrow = spec.find("->");
+    const auto comma = spec.find(',');
+    if (arrow == std::string::npos)
+        throw std::runtime_error("einsum: no ->");
+    lc = spec.substr(arrow + 2);
+    if (comma == std::string::npos || comma > arrow) {
+        la = spec.substr(0, arrow);
+        lb.clear();
+    } else {
+        la = spec.substr(0, comma);
+        lb = spec.substr(comma + 1, arrow - comma - 1);
+    }
+}
+
+// C = beta*C + alpha * einsum(spec, A, B)
+void Engine::einsum(const std::string &spec, const DTensor &A, const DTensor *B, DTensor &C, double alpha,
+                    double beta) {
+    std::string la, lb, lc;
+    parse_spec(spec, la, lb, lc);
+    if (la.size() != A.dims.size() || lc.size() != C.dims.size() || (B && lb.size() != B->dims.size()))
+        throw std::runtime_error("einsum " + spec + ": rank mismatch");
+    auto dim_of = [&](char x) -> int64_t {
+        auto p = la.find(x);
+        if (p != std::string::npos)
+            return A.dims[p];
+        p = lb.find(x);
+        if (B && p != std::string::npos)
+            return B->dims[p];
+        p = lc.find(x);
+        return C.dims[p];
+    };
+    if (!B) {  // pure permutation
+        permute(la, A.p, A.dims, lc, C.p, C.dims, alpha, beta);
+        return;
+    }
+    std::string Lb, Lm, Ln, Lk;
+    for (char x : lc) {
+        const bool ina = la.find(x) != std::string::npos, inb = lb.find(x) != std::string::npos;
+        if (ina && inb)
+            Lb += x;
+        else if (ina)
+            Lm += x;
+        else if (inb)
+            Ln += x;
+        else
+            throw std::runtime_error("einsum " + spec + ": output label not in an operand");
+    }
+    for (char x : la) {
+        const bool inb = lb.find(x) != std::string::npos, inc = lc.find(x) != std::string::npos;
+        if (inb && !inc)
+            Lk += x;
+        else if (!inb && !inc)
+            throw std::runtime_error("einsum " + spec + ": trace over a single operand is not supported");
+    }
+    for (char x : lb)
+        if (la.find(x) == std::string::npos && lc.find(x) == std::string::npos)
+            throw std::runtime_error("einsum " + spec + ": trace over a single operand is not supported");
+    auto prod = [&](const std::string &ls) {
+        int64_t n = 1;
+        for (char x : ls)
+            n *= dim_of(x);
+        return n;
+    };
+    auto dims_of = [&](const std::string &ls) {
+        std::vector<int64_t> d;
+        for (char x : ls)
+            d.push_back(dim_of(x));
+        return d;
+    };
+    const int64_t nb = prod(Lb), M = prod(Lm), N = prod(Ln), K = prod(Lk);
+    // operands in [batch][M][K] and [batch][N][K]
+    DTensor Ap = alloc({nb, M, K}), Bp = alloc({nb, N, K}), Cp = alloc({nb, M, N}, true);
+    permute(la, A.p, A.dims, Lb + Lm + Lk, Ap.p, dims_of(Lb + Lm + Lk), 1.0, 0.0);
+    permute(lb, B->p, B->dims, Lb + Ln + Lk, Bp.p, dims_of(Lb + Ln + Lk), 1.0, 0.0);
+    if (M > 0 && N > 0 && K > 0) {
+        GemmArgs g{};
+        g.nseg = 1;
+        g.M = (int)M;
+        g.N = (int)N;
+        g.alpha = 1.0;
+        g.splits = 1;
+        g.seg[0].K1 = 1;
+        g.seg[0].K2 = (int)K;
+        g.seg[0].A = {Ap.p, K, 0, 1, M * K};
+        g.seg[0].B = {Bp.p, K, 0, 1, N * K};
+        g.C = Cp.p;
+        g.sCm = N;
+        g.sCn = 1;
+        g.sCb = M * N;
+        if (M > 0x7fffffff || N > 0x7fffffff || K > 0x7fffffff || nb > 65535)
+            throw std::runtime_error("einsum " + spec + ": extent too large");
+        check(launch_gemm(g, (int)nb, stream), "einsum gemm");
+    }
+    permute(Lb + Lm + Ln, Cp.p, dims_of(Lb + Lm + Ln), lc, C.p, C.dims, alpha, beta);
+}
+
+double Engine::dot(const double *x, const double *y, int64_t n) {
+    const int nblk = 256;
+    if (!dotbuf)
+        check(hipMalloc((void **)&dotbuf, sizeof(double) * nblk), "hipMalloc dot");
+    hipLaunchKernelGGL(dot_kernel, dim3(nblk), dim3(256), 0, stream, x, y, n, dotbuf);
+    check(hipGetLastError(), "dot");
+    double h[nblk];
+    check(hipMemcpyAsync(h, dotbuf, sizeof(h), hipMemcpyDeviceToHost, stream), "dot copy");
+    check(hipStreamSynchronize(stream), "dot sync");
+    double s = 0.0, c = 0.0;
+    for (int i = 0; i < nblk; i++) {
+        const double u = s + h[i];
+        if (std::fabs(s) >= std::fabs(h[i]))
+            c += (s - u) + h[i];
+        else
+            c += (h[i] - u) + s;
+        s = u;
+    }
+    return s + c;
+}
+
+Engine::~Engine() {
+    if (dotbuf)
+        (void)hipFree(dotbuf);
+}
+
+// ------------------------------------------------------------- CCSD update
+void CcsdState::make_tau(Engine &E, const DTensor &t1, const DTensor &t2, DTensor &tau) {
+    const int64_t n = t2.size();
+    hipLaunchKernelGGL(tau_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, E.stream, tau.p, t2.p, t1.p, o, v);
+    check(hipGetLastError(), "tau");
+}
+
+// R += X_ij^ab + X_ji^ba (scaled)
+static void plus_transpose_add(Engine &E, DTensor &R, const DTensor &X, double alpha) {
+    E.permute("ijab", X.p, X.dims, "ijab", R.p, R.dims, alpha, 1.0);
+    E.permute("jiba", X.p, X.dims, "ijab", R.p, R.dims, alpha, 1.0);
+}
+
+// Particle-particle ladder sum_cd (ac|bd) tau_ij^cd from the Cholesky factor, (a,b) tiles of TAB x TAB
+void CcsdState::ladder_from_factor(Engine &E, const DTensor &tau, DTensor &R) {
+    const int64_t TAB = 16;
+    const int64_t nP = naux;
+    DTensor I = E.alloc({TAB, v, TAB, v});  // I[a][c][b][d] of one tile
+    for (int64_t a0 = 0; a0 < v; a0 += TAB) {
+        const int64_t ta = std::min(TAB, v - a0);
+        for (int64_t b0 = 0; b0 < v; b0 += TAB) {
+            const int64_t tb = std::min(TAB, v - b0);
+            // I[(a,c)][(b,d)] = sum_P L[P][a][c] L[P][b][d]
+            GemmArgs g{};
+            g.nseg = 1;
+            g.M = (int)(ta * v);
+            g.N = (int)(tb * v);
+            g.alpha = 1.0;
+            g.splits = 1;
+            g.seg[0].K1 = 1;
+            g.seg[0].K2 = (int)nP;
+            g.seg[0].A = {Lvv.p + a0 * v, 1, 0, v * v, 0};
+            g.seg[0].B = {Lvv.p + b0 * v, 1, 0, v * v, 0};
+            check(hipMemsetAsync(I.p, 0, sizeof(double) * (size_t)I.size(), E.stream), "memset I");
+            g.C = I.p;
+            g.sCm = tb * v;
+            g.sCn = 1;
+            g.sCb = 0;
+            check(launch_gemm(g, 1, E.stream), "ladder integrals");
+            // R[i][j][a][b] += sum_{c,d} I[a][c][b][d] tau[i][j][c][d]; batch a, rows b, K = (c, d)
+            GemmArgs h{};
+            h.nseg = 1;
+            h.M = (int)tb;
+            h.N = (int)(o * o);
+            h.alpha = 1.0;
+            h.splits = 1;
+            h.seg[0].K1 = (int)v;
+            h.seg[0].K2 = (int)v;
+            h.seg[0].A = {I.p, v, tb * v, 1, v * tb * v};
+            h.seg[0].B = {tau.p, v * v, v, 1, 0};
+            h.C = R.p + a0 * v + b0;
+            h.sCb = v;
+            h.sCm = 1;
+            h.sCn = v * v;
+            check(launch_gemm(h, (int)ta, E.stream), "ladder contraction");
+        }
+    }
+}
+
+void CcsdState::update(Engine &E, DTensor &t1, DTensor &t2) {
+    const int64_t O = o, Vv = v;
+    DTensor tau = E.alloc({O, O, Vv, Vv});
+    make_tau(E, t1, t2, tau);
+    DTensor Foo = E.alloc({O, O}, true), Fvv = E.alloc({Vv, Vv}, true), Fov = E.alloc({O, Vv}, true);
+    E.einsum("kcld,ilcd->ki", L_ovov, &tau, Foo, 1.0, 0.0);
+    E.einsum("kcld,klad->ac", L_ovov, &tau, Fvv, -1.0, 0.0);
+    E.einsum("kcld,ld->kc", L_ovov, &t1, Fov, 1.0, 0.0);
+
+    // -- T1 equation (RCCSD.cpp:117-138)
+    DTensor t1n = E.alloc({O, Vv}, true);
+    E.einsum("ac,ic->ia", Fvv, &t1, t1n, 1.0, 1.0);
+    E.einsum("ki,ka->ia", Foo, &t1, t1n, -1.0, 1.0);
+    E.einsum("kc,kica->ia", Fov, &t2, t1n, 2.0, 1.0);
+    E.einsum("kc,ikca->ia", Fov, &t2, t1n, -1.0, 1.0);
+    {
+        DTensor Fov_t1 = E.alloc({O, O}, true);
+        E.einsum("kc,ic->ki", Fov, &t1, Fov_t1, 1.0, 0.0);
+        E.einsum("ki,ka->ia", Fov_t1, &t1, t1n, 1.0, 1.0);
+    }
+    E.einsum("kcai,kc->ia", ovvo, &t1, t1n, 2.0, 1.0);
+    E.einsum("kiac,kc->ia", oovv, &t1, t1n, -1.0, 1.0);
+    E.einsum("kdac,ikcd->ia", ovvv, &tau, t1n, 2.0, 1.0);
+    E.einsum("kcad,ikcd->ia", ovvv, &tau, t1n, -1.0, 1.0);
+    E.einsum("lcki,klac->ia", ovoo, &tau, t1n, -2.0, 1.0);
+    E.einsum("kcli,klac->ia", ovoo, &tau, t1n, 1.0, 1.0);
+
+    // -- T2 equation
+    DTensor t2n = E.alloc({O, O, Vv, Vv}, true);
+    {
+        // tmp2(a,b,i,c) = -kibc,ka + (ia|cb); abic,jc->ijab, without forming the o v^3 tmp2:
+        //   sum_c (ia|cb) t_j^c  and  -sum_k t_k^a sum_c (ki|bc) t_j^c
+        DTensor X = E.alloc({O, O, Vv, Vv}, true);
+        E.einsum("iacb,jc->ijab", ovvv, &t1, X, 1.0, 0.0);
+        DTensor M = E.alloc({O, O, Vv, O}, true);  // M[k][i][b][j] = sum_c (ki|bc) t_j^c
+        E.einsum("kibc,jc->kibj", oovv, &t1, M, 1.0, 0.0);
+        E.einsum("kibj,ka->ijab", M, &t1, X, -1.0, 1.0);
+        plus_transpose_add(E, t2n, X, 1.0);
+    }
+    {
+        // tmp2(a,k,i,j) = kcai,jc + (ia|jk); akij,kb->ijab
+        DTensor tmp = E.alloc({Vv, O, O, O}, true);
+        E.einsum("kcai,jc->akij", ovvo, &t1, tmp, 1.0, 0.0);
+        E.einsum("iajk->akij", ovoo, nullptr, tmp, 1.0, 1.0);
+        DTensor X = E.alloc({O, O, Vv, Vv}, true);
+        E.einsum("akij,kb->ijab", tmp, &t1, X, 1.0, 0.0);
+        plus_transpose_add(E, t2n, X, -1.0);
+    }
+    E.axpby(t2n.p, t2n.p, oovv_phys.p, 1.0, 1.0, t2n.size());
+    if (t2driver.p)
+        E.axpby(t2n.p, t2n.p, t2driver.p, 1.0, 1.0, t2n.size());
+
+    DTensor Loo = E.alloc({O, O}, true), Lvv = E.alloc({Vv, Vv}, true);
+    E.copy(Loo, Foo);
+    E.einsum("lcki,lc->ki", ovoo, &t1, Loo, 2.0, 1.0);
+    E.einsum("kcli,lc->ki", ovoo, &t1, Loo, -1.0, 1.0);
+    E.copy(Lvv, Fvv);
+    E.einsum("kdac,kd->ac", ovvv, &t1, Lvv, 2.0, 1.0);
+    E.einsum("kcad,kd->ac", ovvv, &t1, Lvv, -1.0, 1.0);
+
+    {
+        DTensor Woooo = E.alloc({O, O, O, O});
+        E.copy(Woooo, oooo_phys);
+        E.einsum("lcki,jc->klij", ovoo, &t1, Woooo, 1.0, 1.0);
+        E.einsum("kclj,ic->klij", ovoo, &t1, Woooo, 1.0, 1.0);
+        E.einsum("kcld,ijcd->klij", ovov, &tau, Woooo, 1.0, 1.0);
+        E.einsum("klij,klab->ijab", Woooo, &tau, t2n, 1.0, 1.0);
+    }
+    // particle-particle ladder (RCCSD.cpp:177-180, :215-216)
+    if (vvvv_phys.p) {
+        DTensor Wvvvv = E.alloc({Vv, Vv, Vv, Vv});
+        E.copy(Wvvvv, vvvv_phys);
+        E.einsum("kdac,kb->abcd", ovvv, &t1, Wvvvv, -1.0, 1.0);
+        E.einsum("kcbd,ka->abcd", ovvv, &t1, Wvvvv, -1.0, 1.0);
+        E.einsum("abcd,ijcd->ijab", Wvvvv, &tau, t2n, 1.0, 1.0);
+    } else {
+        ladder_from_factor(E, tau, t2n);
+        DTensor Z = E.alloc({O, Vv, O, O}, true);  // Z[k][a][i][j] = sum_cd (kd|ac) tau_ij^cd
+        E.einsum("kdac,ijcd->kaij", ovvv, &tau, Z, 1.0, 0.0);
+        E.einsum("kaij,kb->ijab", Z, &t1, t2n, -1.0, 1.0);
+        // sum_cd (kc|bd) tau_ij^cd = Z[k][b][j][i] (tau_ij^cd = tau_ji^dc)
+        E.einsum("kbji,ka->ijab", Z, &t1, t2n, -1.0, 1.0);
+    }
+    {
+        DTensor X = E.alloc({O, O, Vv, Vv}, true);
+        E.einsum("ac,ijcb->ijab", Lvv, &t2, X, 1.0, 0.0);
+        plus_transpose_add(E, t2n, X, 1.0);
+        E.einsum("ki,kjab->ijab", Loo, &t2, X, 1.0, 0.0);
+        plus_transpose_add(E, t2n, X, -1.0);
+    }
+
+    // ring and exchange terms (RCCSD.cpp:182-211, :221-233)
+    DTensor Wvoov = E.alloc({Vv, O, O, Vv}), Wvovo = E.alloc({Vv, O, Vv, O});
+    E.einsum("kcai->akic", ovvo, nullptr, Wvoov, 1.0, 0.0);
+    E.einsum("kcad,id->akic", ovvv, &t1, Wvoov, 1.0, 1.0);
+    E.einsum("kcli,la->akic", ovoo, &t1, Wvoov, -1.0, 1.0);
+    E.einsum("ldkc,ilda->akic", ovov, &t2, Wvoov, -0.5, 1.0);
+    E.einsum("lckd,ilad->akic", ovov, &t2, Wvoov, -0.5, 1.0);
+    E.einsum("ldkc,ilad->akic", ovov, &t2, Wvoov, 1.0, 1.0);
+    {
+        DTensor tmp = E.alloc({O, O, Vv, O}, true);
+        E.einsum("ldkc,id->lkci", ovov, &t1, tmp, 1.0, 0.0);
+        E.einsum("lkci,la->akic", tmp, &t1, Wvoov, -1.0, 1.0);
+    }
+    E.einsum("kiac->akci", oovv, nullptr, Wvovo, 1.0, 0.0);
+    E.einsum("kdac,id->akci", ovvv, &t1, Wvovo, 1.0, 1.0);
+    E.einsum("lcki,la->akci", ovoo, &t1, Wvovo, -1.0, 1.0);
+    E.einsum("lckd,ilda->akci", ovov, &t2, Wvovo, -0.5, 1.0);
+    {
+        DTensor tmp = E.alloc({O, Vv, O, O}, true);
+        E.einsum("lckd,id->lcki", ovov, &t1, tmp, 1.0, 0.0);
+        E.einsum("lcki,la->akci", tmp, &t1, Wvovo, -1.0, 1.0);
+    }
+    {
+        DTensor X = E.alloc({O, O, Vv, Vv}, true);
+        E.einsum("akic,kjcb->ijab", Wvoov, &t2, X, 2.0, 0.0);
+        E.einsum("akci,kjcb->ijab", Wvovo, &t2, X, -1.0, 1.0);
+        plus_transpose_add(E, t2n, X, 1.0);
+        E.einsum("akic,kjbc->ijab", Wvoov, &t2, X, 1.0, 0.0);
+        plus_transpose_add(E, t2n, X, -1.0);
+        E.einsum("bkci,kjac->ijab", Wvovo, &t2, X, 1.0, 0.0);
+        plus_transpose_add(E, t2n, X, -1.0);
+    }
+
+    // Jacobi update with FP64 denominators (RCCSD.cpp:235-236)
+    const int64_t n = t2n.size();
+    hipLaunchKernelGGL(jacobi_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, E.stream, t1n.p, t2n.p, e.p, o,
+                       v);
+    check(hipGetLastError(), "jacobi");
+    t1 = std::move(t1n);
+    t2 = std::move(t2n);
+}
+
+double CcsdState::energy(Engine &E, const DTensor &t1, const DTensor &t2) {
+    // sum_ijab (2 <ij|ab> - <ij|ba>) tau_ij^ab (RCCSD.cpp:85-89) (+ F12 coupling :91-98)
+    DTensor tau = E.alloc({o, o, v, v});
+    make_tau(E, t1, t2, tau);
+    double Ecorr = E.dot(L_oovv_phys.p, tau.p, tau.size());
+    if (f12energy.p) {
+        DTensor Le = E.alloc({o, o, v, v});
+        E.permute("ijab", f12energy.p, f12energy.dims, "ijab", Le.p, Le.dims, 2.0, 0.0);
+        E.permute("ijba", f12energy.p, f12energy.dims, "ijab", Le.p, Le.dims, -1.0, 1.0);
+        Ecorr += E.dot(Le.p, t2.p, t2.size());
+    }
+    return Ecorr;
+}
+
+}  // namespace scess

@@ -1,6 +1,7 @@
 // C ABI of libscess_gpu.so (include/scess_gpu.h): context, resident inputs and
 // the (T) driver that schedules the gfx950 kernels of triples_kernels.hip.
 #include "../../include/scess_gpu.h"
+#include "ccsd_internal.h"
 #include "triples_internal.h"
 
 #include <hip/hip_runtime.h>
@@ -46,6 +47,8 @@ struct scess_ctx {
     void *user = nullptr;
     int64_t nbatch_req = 0;
     TriplesState t;
+    scess::Engine eng;
+    scess::CcsdState cc;
 };
 
 namespace {
@@ -126,6 +129,7 @@ int scess_create(scess_ctx **out, int device) {
         delete c;
         return SCESS_ERR_HIP;
     }
+    c->eng.stream = c->stream;
     *out = c;
     return SCESS_OK;
 }
@@ -136,6 +140,8 @@ void scess_destroy(scess_ctx *c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     free_triples(c->t);
+    c->cc = scess::CcsdState{};
+    (void)hipStreamSynchronize(c->stream);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -775,6 +781,305 @@ int scess_t_backward(scess_ctx *c, int prec, double dE, scess_t_grad *grad) {
     if (rc == SCESS_OK && cancelled)
         rc = fail(c, SCESS_CANCELLED, "scess_t_backward: cancelled");
     return rc;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- CCSD ABI
+namespace {
+
+int engine_fail(scess_ctx *c, const std::exception &ex) {
+    const std::string m = ex.what();
+    const int code = m.find("out of memory") != std::string::npos ? SCESS_ERR_OOM : SCESS_ERR_HIP;
+    return fail(c, code, m);
+}
+
+// ColMajor host tensor (labels in index order) -> row-major device tensor with the same label order
+scess::DTensor upload_colmajor(scess::Engine &E, const double *h, std::vector<int64_t> dims) {
+    std::vector<int64_t> rev(dims.rbegin(), dims.rend());
+    scess::DTensor raw = E.alloc(rev);
+    E.upload(raw, h);
+    scess::DTensor out = E.alloc(dims);
+    std::string lab = std::string("abcdefgh").substr(0, dims.size());
+    std::string rl(lab.rbegin(), lab.rend());
+    E.permute(rl, raw.p, raw.dims, lab, out.p, out.dims, 1.0, 0.0);
+    return out;
+}
+
+void download_colmajor(scess::Engine &E, const scess::DTensor &t, double *h) {
+    std::vector<int64_t> rev(t.dims.rbegin(), t.dims.rend());
+    scess::DTensor tmp = E.alloc(rev);
+    std::string lab = std::string("abcdefgh").substr(0, t.dims.size());
+    std::string rl(lab.rbegin(), lab.rend());
+    E.permute(lab, t.p, t.dims, rl, tmp.p, tmp.dims, 1.0, 0.0);
+    E.download(tmp, h);
+}
+
+// Solve the bordered DIIS system with full pivoting (Eigen's fullPivLu in the reference)
+std::vector<double> solve_full_pivot(std::vector<double> A, std::vector<double> b, int n) {
+    std::vector<int> cp(n);
+    for (int i = 0; i < n; i++)
+        cp[i] = i;
+    for (int k = 0; k < n; k++) {
+        int pr = k, pc = k;
+        double best = -1.0;
+        for (int i = k; i < n; i++)
+            for (int j = k; j < n; j++)
+                if (std::fabs(A[i * n + j]) > best) {
+                    best = std::fabs(A[i * n + j]);
+                    pr = i;
+                    pc = j;
+                }
+        if (best == 0.0)
+            break;
+        if (pr != k) {
+            for (int j = 0; j < n; j++)
+                std::swap(A[k * n + j], A[pr * n + j]);
+            std::swap(b[k], b[pr]);
+        }
+        if (pc != k) {
+            for (int i = 0; i < n; i++)
+                std::swap(A[i * n + k], A[i * n + pc]);
+            std::swap(cp[k], cp[pc]);
+        }
+        for (int i = k + 1; i < n; i++) {
+            const double f = A[i * n + k] / A[k * n + k];
+            for (int j = k; j < n; j++)
+                A[i * n + j] -= f * A[k * n + j];
+            b[i] -= f * b[k];
+        }
+    }
+    std::vector<double> y(n, 0.0), x(n, 0.0);
+    for (int k = n - 1; k >= 0; k--) {
+        if (A[k * n + k] == 0.0) {
+            y[k] = 0.0;
+            continue;
+        }
+        double s2 = b[k];
+        for (int j = k + 1; j < n; j++)
+            s2 -= A[k * n + j] * y[j];
+        y[k] = s2 / A[k * n + k];
+    }
+    for (int k = 0; k < n; k++)
+        x[cp[k]] = y[k];
+    return x;
+}
+
+}  // namespace
+
+extern "C" {
+
+int scess_ccsd_set_inputs(scess_ctx *c, const scess_ccsd_in *in) {
+    if (!c)
+        return SCESS_ERR_ARG;
+    if (!in || in->o < 1 || in->v < 1 || !in->e || !in->ovov || !in->oovv || !in->ovvo || !in->ovvv || !in->ovoo ||
+        !in->oooo || (!in->vvvv && !in->chol_vv) || (in->chol_vv && in->naux < 1))
+        return fail(c, SCESS_ERR_ARG, "scess_ccsd_set_inputs: missing block or non-positive size");
+    HIP_TRY(c, hipSetDevice(c->device));
+    try {
+        scess::Engine &E = c->eng;
+        scess::CcsdState &S = c->cc;
+        S = scess::CcsdState{};
+        const int64_t o = in->o, v = in->v;
+        S.o = o;
+        S.v = v;
+        S.ovov = upload_colmajor(E, in->ovov, {o, v, o, v});
+        S.oovv = upload_colmajor(E, in->oovv, {o, o, v, v});
+        S.ovvo = upload_colmajor(E, in->ovvo, {o, v, v, o});
+        S.ovvv = upload_colmajor(E, in->ovvv, {o, v, v, v});
+        S.ovoo = upload_colmajor(E, in->ovoo, {o, v, o, o});
+        {
+            scess::DTensor oooo = upload_colmajor(E, in->oooo, {o, o, o, o});  // (k,i,l,j)
+            S.oooo_phys = E.alloc({o, o, o, o});
+            E.einsum("kilj->klij", oooo, nullptr, S.oooo_phys, 1.0, 0.0);
+        }
+        S.oovv_phys = E.alloc({o, o, v, v});
+        E.einsum("iajb->ijab", S.ovov, nullptr, S.oovv_phys, 1.0, 0.0);
+        S.L_oovv_phys = E.alloc({o, o, v, v});
+        E.einsum("ijab->ijab", S.oovv_phys, nullptr, S.L_oovv_phys, 2.0, 0.0);
+        E.einsum("ijba->ijab", S.oovv_phys, nullptr, S.L_oovv_phys, -1.0, 1.0);
+        S.L_ovov = E.alloc({o, v, o, v});
+        E.einsum("kcld->kcld", S.ovov, nullptr, S.L_ovov, 2.0, 0.0);
+        E.einsum("kdlc->kcld", S.ovov, nullptr, S.L_ovov, -1.0, 1.0);
+        if (in->vvvv) {
+            scess::DTensor vvvv = upload_colmajor(E, in->vvvv, {v, v, v, v});  // (a,c,b,d)
+            S.vvvv_phys = E.alloc({v, v, v, v});
+            E.einsum("acbd->abcd", vvvv, nullptr, S.vvvv_phys, 1.0, 0.0);
+        } else {
+            S.naux = in->naux;
+            scess::DTensor L = upload_colmajor(E, in->chol_vv, {v, v, in->naux});  // (a,c,P)
+            S.Lvv = E.alloc({in->naux, v, v});
+            E.einsum("acP->Pac", L, nullptr, S.Lvv, 1.0, 0.0);
+        }
+        S.e = E.alloc({o + v});
+        E.upload(S.e, in->e);
+        if (in->t2_driver)
+            S.t2driver = upload_colmajor(E, in->t2_driver, {o, o, v, v});
+        if (in->f12_energy)
+            S.f12energy = upload_colmajor(E, in->f12_energy, {o, o, v, v});
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        S.have = true;
+    } catch (const std::exception &ex) {
+        c->cc = scess::CcsdState{};
+        return engine_fail(c, ex);
+    }
+    return SCESS_OK;
+}
+
+int scess_ccsd_update(scess_ctx *c, const double *t1, const double *t2, double *t1_new, double *t2_new) {
+    if (!c)
+        return SCESS_ERR_ARG;
+    if (!t1 || !t2 || !t1_new || !t2_new)
+        return fail(c, SCESS_ERR_ARG, "scess_ccsd_update: null array");
+    if (!c->cc.have)
+        return fail(c, SCESS_ERR_STATE, "scess_ccsd_update: no inputs (call scess_ccsd_set_inputs first)");
+    HIP_TRY(c, hipSetDevice(c->device));
+    try {
+        scess::Engine &E = c->eng;
+        scess::CcsdState &S = c->cc;
+        scess::DTensor d1 = upload_colmajor(E, t1, {S.o, S.v});
+        scess::DTensor d2 = upload_colmajor(E, t2, {S.o, S.o, S.v, S.v});
+        S.update(E, d1, d2);
+        download_colmajor(E, d1, t1_new);
+        download_colmajor(E, d2, t2_new);
+    } catch (const std::exception &ex) {
+        return engine_fail(c, ex);
+    }
+    return SCESS_OK;
+}
+
+int scess_ccsd_energy(scess_ctx *c, const double *t1, const double *t2, double *e_corr) {
+    if (!c)
+        return SCESS_ERR_ARG;
+    if (!t1 || !t2 || !e_corr)
+        return fail(c, SCESS_ERR_ARG, "scess_ccsd_energy: null argument");
+    if (!c->cc.have)
+        return fail(c, SCESS_ERR_STATE, "scess_ccsd_energy: no inputs");
+    HIP_TRY(c, hipSetDevice(c->device));
+    try {
+        scess::Engine &E = c->eng;
+        scess::CcsdState &S = c->cc;
+        scess::DTensor d1 = upload_colmajor(E, t1, {S.o, S.v});
+        scess::DTensor d2 = upload_colmajor(E, t2, {S.o, S.o, S.v, S.v});
+        *e_corr = S.energy(E, d1, d2);
+    } catch (const std::exception &ex) {
+        return engine_fail(c, ex);
+    }
+    return SCESS_OK;
+}
+
+int scess_ccsd_solve(scess_ctx *c, double *t1_out, double *t2_out, scess_ccsd_result *res, scess_ccsd_log log,
+                     void *user) {
+    if (!c)
+        return SCESS_ERR_ARG;
+    if (!res)
+        return fail(c, SCESS_ERR_ARG, "scess_ccsd_solve: null result");
+    std::memset(res, 0, sizeof(*res));
+    if (!c->cc.have)
+        return fail(c, SCESS_ERR_STATE, "scess_ccsd_solve: no inputs");
+    HIP_TRY(c, hipSetDevice(c->device));
+    // thresholds of SolveRCCSD_DIIS (RCCSD.cpp:302-307)
+    constexpr double ENERGY_CONVERGENCE = 1.0e-10, AMPLITUDE_CONVERGENCE = 1.0e-8;
+    constexpr int MAX_ITERATIONS = 100, DIIS_START = 2, MAX_DIIS_VECS = 8;
+    try {
+        scess::Engine &E = c->eng;
+        scess::CcsdState &S = c->cc;
+        const int64_t o = S.o, v = S.v, n1 = o * v, n2 = o * o * v * v, nt = n1 + n2;
+        // flattened amplitudes [t1 | t2] (device row-major order); t1/t2 view the current vector
+        scess::DTensor t1 = E.alloc({o, v}, true), t2 = E.alloc({o, o, v, v});
+        // MP2 guess (RCCSD.cpp:333-339)
+        E.copy(t2, S.oovv_phys);
+        if (S.t2driver.p)
+            E.axpby(t2.p, t2.p, S.t2driver.p, 1.0, 1.0, n2);
+        {
+            // t2 = (<ij|ab> + driver) / D with the update's Jacobi kernel (t1 stays 0)
+            scess::DTensor z1 = E.alloc({o, v}, true);
+            HIP_TRY(c, scess::ccsd_jacobi(z1.p, t2.p, S.e.p, (int)o, (int)v, c->stream));
+        }
+        std::vector<scess::DTensor> amps, errs;
+        double E_last = 1.7976931348623157e308;
+        bool converged = false;
+        int it = 0;
+        scess::DTensor prev = E.alloc({nt}), upd = E.alloc({nt}), change = E.alloc({nt});
+        for (it = 0; it < MAX_ITERATIONS; it++) {
+            if (c->should_stop && c->should_stop(c->user)) {
+                res->iterations = it;
+                return fail(c, SCESS_CANCELLED, "scess_ccsd_solve: cancelled");
+            }
+            HIP_TRY(c, hipMemcpyAsync(prev.p, t1.p, 8 * n1, hipMemcpyDeviceToDevice, c->stream));
+            HIP_TRY(c, hipMemcpyAsync(prev.p + n1, t2.p, 8 * n2, hipMemcpyDeviceToDevice, c->stream));
+            S.update(E, t1, t2);
+            HIP_TRY(c, hipMemcpyAsync(upd.p, t1.p, 8 * n1, hipMemcpyDeviceToDevice, c->stream));
+            HIP_TRY(c, hipMemcpyAsync(upd.p + n1, t2.p, 8 * n2, hipMemcpyDeviceToDevice, c->stream));
+            E.axpby(change.p, upd.p, prev.p, 1.0, -1.0, nt);
+            // AmplitudeDIIS::Extrapolate (RCCSD.cpp:246-274)
+            {
+                scess::DTensor a = E.alloc({nt}), er = E.alloc({nt});
+                E.copy(a, upd);
+                E.copy(er, change);
+                amps.push_back(std::move(a));
+                errs.push_back(std::move(er));
+                if ((int)errs.size() > MAX_DIIS_VECS) {
+                    amps.erase(amps.begin());
+                    errs.erase(errs.begin());
+                }
+            }
+            if (it >= DIIS_START) {
+                const int n = (int)errs.size();
+                std::vector<double> B((n + 1) * (n + 1), -1.0), rhs(n + 1, 0.0);
+                B[n * (n + 1) + n] = 0.0;
+                for (int i = 0; i < n; i++)
+                    for (int j = 0; j <= i; j++) {
+                        const double d = E.dot(errs[i].p, errs[j].p, nt);
+                        B[i * (n + 1) + j] = B[j * (n + 1) + i] = d;
+                    }
+                double mx = 0.0;
+                for (int i = 0; i < n; i++)
+                    for (int j = 0; j < n; j++)
+                        mx = std::max(mx, std::fabs(B[i * (n + 1) + j]));
+                if (mx > 0.0)
+                    for (int i = 0; i < n; i++)
+                        for (int j = 0; j < n; j++)
+                            B[i * (n + 1) + j] /= mx;
+                rhs[n] = -1.0;
+                const std::vector<double> coef = solve_full_pivot(B, rhs, n + 1);
+                HIP_TRY(c, hipMemsetAsync(upd.p, 0, 8 * nt, c->stream));
+                for (int i = 0; i < n; i++)
+                    E.axpby(upd.p, upd.p, amps[i].p, 1.0, coef[i], nt);
+                HIP_TRY(c, hipMemcpyAsync(t1.p, upd.p, 8 * n1, hipMemcpyDeviceToDevice, c->stream));
+                HIP_TRY(c, hipMemcpyAsync(t2.p, upd.p + n1, 8 * n2, hipMemcpyDeviceToDevice, c->stream));
+            }
+            const double Ecur = S.energy(E, t1, t2);
+            const double dE = std::fabs(Ecur - E_last);
+            const double rms = std::sqrt(std::max(0.0, E.dot(change.p, change.p, nt))) / std::sqrt((double)nt);
+            E_last = Ecur;
+            res->last_dE = dE;
+            res->last_rms = rms;
+            if (log)
+                log(user, it, Ecur, dE, rms);
+            if (dE < ENERGY_CONVERGENCE && rms < AMPLITUDE_CONVERGENCE) {
+                converged = true;
+                break;
+            }
+        }
+        res->converged = converged ? 1 : 0;
+        res->iterations = converged ? it + 1 : it;
+        res->E_CCSD = E_last;
+        if (S.f12energy.p) {
+            scess::DTensor Le = E.alloc({o, o, v, v});
+            E.permute("ijab", S.f12energy.p, S.f12energy.dims, "ijab", Le.p, Le.dims, 2.0, 0.0);
+            E.permute("ijba", S.f12energy.p, S.f12energy.dims, "ijab", Le.p, Le.dims, -1.0, 1.0);
+            res->E_F12_Coupling = E.dot(Le.p, t2.p, n2);
+        }
+        if (t1_out)
+            download_colmajor(E, t1, t1_out);
+        if (t2_out)
+            download_colmajor(E, t2, t2_out);
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    } catch (const std::exception &ex) {
+        return engine_fail(c, ex);
+    }
+    return SCESS_OK;
 }
 
 }  // extern "C"

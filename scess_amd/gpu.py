@@ -47,7 +47,19 @@ class TGrad(ctypes.Structure):
     _fields_ = [("ovvv", _dp), ("ovoo", _dp), ("ovov", _dp), ("t1", _dp), ("t2", _dp)]
 
 
+class CcsdIn(ctypes.Structure):
+    _fields_ = [("o", ctypes.c_int64), ("v", ctypes.c_int64), ("e", _dp), ("ovov", _dp), ("oovv", _dp),
+                ("ovvo", _dp), ("ovvv", _dp), ("ovoo", _dp), ("oooo", _dp), ("vvvv", _dp), ("chol_vv", _dp),
+                ("naux", ctypes.c_int64), ("t2_driver", _dp), ("f12_energy", _dp)]
+
+
+class CcsdResult(ctypes.Structure):
+    _fields_ = [("E_CCSD", ctypes.c_double), ("E_F12_Coupling", ctypes.c_double), ("converged", ctypes.c_int),
+                ("iterations", ctypes.c_int), ("last_dE", ctypes.c_double), ("last_rms", ctypes.c_double)]
+
+
 STOP_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p)
+LOG_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double)
 
 _LIB = None
 
@@ -95,9 +107,16 @@ def load(path: str = LIB_PATH):
     lib.scess_t_num_ijk.restype = c.c_int64
     lib.scess_t_num_partials.argtypes = [c.c_void_p]
     lib.scess_t_num_partials.restype = c.c_int64
-    if hasattr(lib, "scess_t_backward"):
-        lib.scess_t_backward.argtypes = [c.c_void_p, c.c_int, c.c_double, c.POINTER(TGrad)]
-        lib.scess_t_backward.restype = c.c_int
+    lib.scess_t_backward.argtypes = [c.c_void_p, c.c_int, c.c_double, c.POINTER(TGrad)]
+    lib.scess_t_backward.restype = c.c_int
+    lib.scess_ccsd_set_inputs.argtypes = [c.c_void_p, c.POINTER(CcsdIn)]
+    lib.scess_ccsd_set_inputs.restype = c.c_int
+    lib.scess_ccsd_update.argtypes = [c.c_void_p, _dp, _dp, _dp, _dp]
+    lib.scess_ccsd_update.restype = c.c_int
+    lib.scess_ccsd_energy.argtypes = [c.c_void_p, _dp, _dp, _dp]
+    lib.scess_ccsd_energy.restype = c.c_int
+    lib.scess_ccsd_solve.argtypes = [c.c_void_p, _dp, _dp, c.POINTER(CcsdResult), LOG_FN, c.c_void_p]
+    lib.scess_ccsd_solve.restype = c.c_int
     _LIB = lib
     return lib
 
@@ -205,3 +224,45 @@ class Context:
         tg = TGrad(*[_ptr(g[k]) for k in ("ovvv", "ovoo", "ovov", "t1", "t2")])
         self.check(self.lib.scess_t_backward(self.h, PRECISIONS[precision], float(dE), ctypes.byref(tg)))
         return g
+
+    # ------------------------------------------------------------------ CCSD
+    def set_ccsd_inputs(self, o, v, e, blocks, chol_vv=None, t2_driver=None, f12_energy=None):
+        """blocks: dict of chemists' blocks (ovov, oovv, ovvo, ovvv, ovoo, oooo and optionally vvvv),
+        logical index order as in RCCSD.cpp:31-44; chol_vv: (a, c, P) factor of (ac|bd)."""
+        keep = {k: _f64(blocks[k]) for k in ("ovov", "oovv", "ovvo", "ovvv", "ovoo", "oooo")}
+        keep["e"] = _f64(e)
+        vvvv = _f64(blocks["vvvv"]) if blocks.get("vvvv") is not None else None
+        ch = _f64(chol_vv) if chol_vv is not None else None
+        drv = _f64(t2_driver) if t2_driver is not None else None
+        fen = _f64(f12_energy) if f12_energy is not None else None
+        cin = CcsdIn(int(o), int(v), _ptr(keep["e"]), _ptr(keep["ovov"]), _ptr(keep["oovv"]), _ptr(keep["ovvo"]),
+                     _ptr(keep["ovvv"]), _ptr(keep["ovoo"]), _ptr(keep["oooo"]),
+                     _ptr(vvvv) if vvvv is not None else None, _ptr(ch) if ch is not None else None,
+                     int(ch.shape[2]) if ch is not None else 0,
+                     _ptr(drv) if drv is not None else None, _ptr(fen) if fen is not None else None)
+        self.check(self.lib.scess_ccsd_set_inputs(self.h, ctypes.byref(cin)))
+        self.cc_o, self.cc_v = int(o), int(v)
+
+    def ccsd_update(self, t1, t2):
+        t1 = _f64(t1)
+        t2 = _f64(t2)
+        n1 = np.zeros(t1.shape, order="F")
+        n2 = np.zeros(t2.shape, order="F")
+        self.check(self.lib.scess_ccsd_update(self.h, _ptr(t1), _ptr(t2), _ptr(n1), _ptr(n2)))
+        return n1, n2
+
+    def ccsd_energy(self, t1, t2):
+        t1 = _f64(t1)
+        t2 = _f64(t2)
+        e = ctypes.c_double(0.0)
+        self.check(self.lib.scess_ccsd_energy(self.h, _ptr(t1), _ptr(t2), ctypes.byref(e)))
+        return e.value
+
+    def ccsd_solve(self, log=None):
+        o, v = self.cc_o, self.cc_v
+        t1 = np.zeros((o, v), order="F")
+        t2 = np.zeros((o, o, v, v), order="F")
+        res = CcsdResult()
+        cb = LOG_FN(lambda _u, it, E, dE, rms: log(it, E, dE, rms)) if log else LOG_FN(0)
+        self.check(self.lib.scess_ccsd_solve(self.h, _ptr(t1), _ptr(t2), ctypes.byref(res), cb, None))
+        return {k: getattr(res, k) for k, _ in res._fields_} | {"t1": t1, "t2": t2}

@@ -1,0 +1,123 @@
+"""Parity of the gfx950 CCSD residual engine (scess_ccsd_*) with the CCSD oracle
+(oracle/ccsd_oracle.py, restating src/Quantum/CC/RCCSD.cpp).
+
+Tolerances: one UpdateAmplitudes step agrees to 1e-11 relative to the largest
+amplitude (FP64, different summation order); converged correlation energies to
+1e-9 Eh (the solver's own convergence threshold is 1e-10)."""
+
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import ccsd_oracle as co
+from scess_amd import gpu
+from scess_amd.synthetic import _draw, full_eri_chem
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _system(o, v, seed, scale):
+    e, L, _ = _draw(o, v, seed)
+    L = L * scale
+    eri = full_eri_chem(L)
+    O, V = slice(0, o), slice(o, None)
+    blocks = {"ovov": eri[O, V, O, V], "oovv": eri[O, O, V, V], "ovvo": eri[O, V, V, O],
+              "ovvv": eri[O, V, V, V], "ovoo": eri[O, V, O, O], "oooo": eri[O, O, O, O],
+              "vvvv": eri[V, V, V, V]}
+    chol = np.asfortranarray(L[:, o:, o:].transpose(1, 2, 0))
+    return eri, e, blocks, chol
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = gpu.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("o,v,seed,use_chol", [(2, 3, 1, False), (3, 7, 2, False), (3, 7, 2, True),
+                                               (5, 20, 3, True), (4, 37, 4, True)])
+def test_update_matches_oracle(ctx, o, v, seed, use_chol):
+    _say(f"case ccsd update o={o} v={v} chol={use_chol}")
+    eri, e, blocks, chol = _system(o, v, seed, 0.3)
+    if use_chol:
+        blocks = dict(blocks, vvvv=None)
+    ctx.set_ccsd_inputs(o, v, e, blocks, chol_vv=chol if use_chol else None)
+    rng = np.random.default_rng(seed)
+    t1 = rng.standard_normal((o, v)) * 1e-2
+    t2 = rng.standard_normal((o, o, v, v)) * 1e-2
+    t2 = t2 + t2.transpose(1, 0, 3, 2)
+    g1, g2 = ctx.ccsd_update(t1, t2)
+    ints = co.make_blocks(eri, o)
+    eia, eijab = co.denominators(e, o)
+    r1, r2 = co.update_amplitudes(ints, eia, eijab, None, t1, t2)
+    assert np.abs(g1 - r1).max() <= 1e-11 * np.abs(r1).max()
+    assert np.abs(g2 - r2).max() <= 1e-11 * np.abs(r2).max()
+    assert abs(ctx.ccsd_energy(t1, t2) - co.correlation_energy(ints, t1, t2)) < 1e-12 * max(1.0, abs(
+        co.correlation_energy(ints, t1, t2)))
+
+
+@pytest.mark.parametrize("o,v,seed,use_chol", [(2, 4, 2, False), (3, 8, 3, True)])
+def test_solve_matches_oracle(ctx, o, v, seed, use_chol):
+    _say(f"case ccsd solve o={o} v={v} chol={use_chol}")
+    eri, e, blocks, chol = _system(o, v, seed, 0.25)
+    if use_chol:
+        blocks = dict(blocks, vvvv=None)
+    ctx.set_ccsd_inputs(o, v, e, blocks, chol_vv=chol if use_chol else None)
+    r = ctx.ccsd_solve()
+    ro = co.solve_rccsd_diis(eri, e, o)
+    assert r["converged"] and ro["converged"]
+    assert abs(r["E_CCSD"] - ro["E_CCSD"]) < 1e-9
+    assert np.abs(r["t2"] - ro["t2"]).max() < 1e-7
+
+
+def test_f12_driver_and_coupling(ctx):
+    _say("case ccsd f12 driver")
+    o, v = 2, 3
+    eri, e, blocks, chol = _system(o, v, 6, 0.3)
+    rng = np.random.default_rng(1)
+    res = rng.standard_normal((o, o, v, v)) * 1e-3
+    res = res + res.transpose(1, 0, 3, 2)
+    en = rng.standard_normal((o, o, v, v)) * 1e-3
+    ctx.set_ccsd_inputs(o, v, e, blocks, t2_driver=res, f12_energy=en)
+    r = ctx.ccsd_solve()
+    ro = co.solve_rccsd_diis(eri, e, o, f12={"residual": res, "energy": en})
+    assert r["converged"] and abs(r["E_CCSD"] - ro["E_CCSD"]) < 1e-9
+    assert abs(r["E_F12_Coupling"] - co.coupling_energy({"energy": en}, ro["t2"])) < 1e-9
+
+
+@pytest.mark.parametrize("basis", ["STO-3G", "cc-pVDZ"])
+def test_water_end_to_end_on_gpu(ctx, basis):
+    """Water at the reference's built-in geometry: CCSD (scess_ccsd_solve) and (T)
+    (scess_t_forward, FP64) on the GPU against the published energies
+    (Tests.cpp:43-44, 52-53) and against the CPU oracles (1e-10 Eh)."""
+    _say(f"case water {basis}")
+    import oracle
+    from oracle import scf
+    from scess_amd.synthetic import TriplesInputs
+    ref = {"STO-3G": (-0.0494026324, -0.0000673499, 1e-7, 1e-8),
+           "cc-pVDZ": (-0.2133077024, -0.0030574132, 1e-7, 1e-7)}[basis]
+    r = scf.molecule_inputs(basis)
+    o, eri, e = r["nocc"], r["eri"], r["e"]
+    v = e.size - o
+    O, V = slice(0, o), slice(o, None)
+    blocks = {"ovov": eri[O, V, O, V], "oovv": eri[O, O, V, V], "ovvo": eri[O, V, V, O], "ovvv": eri[O, V, V, V],
+              "ovoo": eri[O, V, O, O], "oooo": eri[O, O, O, O], "vvvv": eri[V, V, V, V]}
+    ctx.set_ccsd_inputs(o, v, e, blocks)
+    cc = ctx.ccsd_solve()
+    assert cc["converged"]
+    assert abs(cc["E_CCSD"] - ref[0]) <= ref[2]
+    cco = co.solve_rccsd_diis(eri, e, o)
+    assert abs(cc["E_CCSD"] - cco["E_CCSD"]) < 1e-10
+    inp = TriplesInputs(o=o, v=v, e=e, t1=cc["t1"], t2=cc["t2"], ovvv=np.asfortranarray(eri[O, V, V, V]),
+                        ovoo=np.asfortranarray(eri[O, V, O, O]), ovov=np.asfortranarray(eri[O, V, O, V]))
+    ctx.set_triples_inputs(inp, ("fp64",))
+    ET, _ = ctx.triples_forward("fp64")
+    assert abs(ET - ref[1]) <= ref[3]
+    assert abs(ET - oracle.t_abc(inp)) < 1e-10
