@@ -5,8 +5,8 @@
  * The reference has no C ABI: its boundary is the C++ method API of
  * libQuantumSCESS, src/Quantum/CC/CC.h:136-164. The entry points below are what
  * a binding of that API to this library calls; the C++ mirror of CC.h in
- * scess_amd/csrc/cc_api.{h,cpp} and the Python mirror in scess_amd/triples.py
- * sit on top of them.
+ * include/scess_cc.h (scess_amd/csrc/cc_api.cpp) and the Python mirror in
+ * scess_amd/triples.py sit on top of them.
  *
  * Conventions
  *   - All host arrays are FP64 in Eigen's default ColMajor layout (first index
@@ -167,6 +167,12 @@ int scess_ccsd_energy(scess_ctx *ctx, const double *t1, const double *t2, double
  * be NULL. */
 int scess_ccsd_solve(scess_ctx *ctx, double *t1_out, double *t2_out, scess_ccsd_result *res, scess_ccsd_log log,
                      void *user);
+
+/* TransformTEI (IntegralTransforms.cpp:29-59): (pq|rs) = sum C(mu,p) C(nu,q) C(kap,r) C(lam,s) (mu nu|kap lam),
+ * one index at a time on the GPU (four FP64 GEMMs). tei is the nao^4 AO tensor in chemists' notation, C is
+ * nao x nmo, out is nmo^4; with chem_to_phys != 0, out(p,r,q,s) = (pq|rs) = <pr|qs>. ColMajor throughout. */
+int scess_transform_tei(scess_ctx *ctx, int64_t nao, int64_t nmo, const double *tei, const double *C,
+                        int chem_to_phys, double *out);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,195 @@
+// SCESS::CC method API (include/scess_cc.h) over the C ABI (include/scess_gpu.h).
+#include "../../include/scess_cc.h"
+#include "../../include/scess_gpu.h"
+
+#include <iomanip>
+#include <memory>
+#include <stdexcept>
+#include <string>
+
+namespace SCESS::CC {
+namespace {
+
+struct Ctx {
+    scess_ctx *c = nullptr;
+    Ctx() {
+        if (scess_create(&c, 0) != SCESS_OK)
+            throw std::runtime_error("SCESS: no usable GPU (scess_create failed)");
+    }
+    ~Ctx() { scess_destroy(c); }
+    void check(int rc) const {
+        if (rc != SCESS_OK && rc != SCESS_CANCELLED)
+            throw std::runtime_error(std::string("SCESS: ") + scess_last_error(c));
+    }
+};
+
+int stop_cb(void *u) { return u && static_cast<const Thread *>(u)->ShouldStop() ? 1 : 0; }
+
+// A ColMajor sub-block of <pq|rs>: blk(i0,i1,i2,i3) = eri(r[0]+i_{p0}, ...), with the block's
+// index k reading eri's argument pos[k]. Returns the block in chemists' naming order.
+template <class F>
+std::vector<double> block(int64_t d0, int64_t d1, int64_t d2, int64_t d3, F f) {
+    std::vector<double> out((size_t)(d0 * d1 * d2 * d3));
+    size_t n = 0;
+    for (int64_t l = 0; l < d3; l++)
+        for (int64_t k = 0; k < d2; k++)
+            for (int64_t j = 0; j < d1; j++)
+                for (int64_t i = 0; i < d0; i++)
+                    out[n++] = f(i, j, k, l);
+    return out;
+}
+
+}  // namespace
+
+RCCSD_Wfn run_ccsd_t(const TEITensor &eri, const std::vector<double> &e, int64_t nOcc, double E_HF,
+                     const Thread *thread, std::stringstream *console);
+
+RCCSD_Wfn SolveRCCSD_DIIS(const TEITensor &eri, const std::vector<double> &e, int64_t nOcc,
+                          const RCCSD_F12Coupling *f12, const Thread *thread, std::stringstream *console) {
+    const int64_t o = nOcc, n = eri.dimension(0), v = n - o;
+    if (console)
+        *console << std::setprecision(12)
+                 << (f12 ? "--- Closed-shell CCSD with F12 geminals (CCSD-F12b) with DIIS acceleration ---\n"
+                         : "--- Closed-shell Coupled Cluster Singles & Doubles (CCSD) with DIIS acceleration ---\n");
+    // chemists' blocks from <pq|rs> = (pr|qs), as MakeIntegralBlocks (RCCSD.cpp:47-72)
+    const auto ovov = block(o, v, o, v, [&](int64_t k, int64_t c, int64_t l, int64_t d) { return eri(k, l, o + c, o + d); });
+    const auto oovv = block(o, o, v, v, [&](int64_t k, int64_t i, int64_t a, int64_t c) { return eri(k, o + a, i, o + c); });
+    const auto ovvo = block(o, v, v, o, [&](int64_t k, int64_t c, int64_t a, int64_t i) { return eri(k, o + a, o + c, i); });
+    const auto ovvv = block(o, v, v, v, [&](int64_t k, int64_t d, int64_t a, int64_t c) { return eri(k, o + a, o + d, o + c); });
+    const auto ovoo = block(o, v, o, o, [&](int64_t l, int64_t c, int64_t k, int64_t i) { return eri(l, k, o + c, i); });
+    const auto oooo = block(o, o, o, o, [&](int64_t k, int64_t i, int64_t l, int64_t j) { return eri(k, l, i, j); });
+    const auto vvvv = block(v, v, v, v, [&](int64_t a, int64_t c, int64_t b, int64_t d) { return eri(o + a, o + b, o + c, o + d); });
+    Ctx ctx;
+    scess_ccsd_in in{};
+    in.o = o;
+    in.v = v;
+    in.e = e.data();
+    in.ovov = ovov.data();
+    in.oovv = oovv.data();
+    in.ovvo = ovvo.data();
+    in.ovvv = ovvv.data();
+    in.ovoo = ovoo.data();
+    in.oooo = oooo.data();
+    in.vvvv = vvvv.data();
+    in.t2_driver = f12 ? f12->residual.data() : nullptr;
+    in.f12_energy = f12 ? f12->energy.data() : nullptr;
+    ctx.check(scess_ccsd_set_inputs(ctx.c, &in));
+    ctx.check(scess_set_should_stop(ctx.c, stop_cb, (void *)thread));
+    RCCSD_Wfn wfn;
+    T1Tensor t1(o, v);
+    T2Tensor t2(o, o, v, v);
+    scess_ccsd_result res{};
+    struct Log {
+        std::stringstream *con;
+    } lg{console};
+    auto log = [](void *u, int it, double E, double dE, double rms) {
+        auto *l = static_cast<Log *>(u);
+        if (!l->con)
+            return;
+        if (it == 2)
+            *l->con << "Turning on DIIS\n";
+        *l->con << "   CCSD(" << it << "): E(corr) = " << E << "      dE = " << dE << "      rms(dt) = " << rms << '\n';
+    };
+    const int rc = scess_ccsd_solve(ctx.c, t1.data(), t2.data(), &res, log, &lg);
+    if (rc == SCESS_CANCELLED)
+        return {};
+    ctx.check(rc);
+    if (!res.converged) {
+        if (console)
+            *console << "!!! CCSD procedure failed to converge !!!\n\n\n";
+        return wfn;
+    }
+    wfn.converged = true;
+    wfn.E_CCSD = res.E_CCSD;
+    wfn.E_F12_Coupling = res.E_F12_Coupling;
+    wfn.t1 = std::move(t1);
+    wfn.t2 = std::move(t2);
+    if (console) {
+        *console << "---- CCSD procedure converged! ----\n"
+                 << "E(CCSD correlation) = " << wfn.E_CCSD << " hartrees\n";
+        if (f12)
+            *console << "   of which E(T2-geminal coupling) = " << wfn.E_F12_Coupling << " hartrees\n";
+    }
+    return wfn;
+}
+
+double RCCSD_TriplesCorrection(const TEITensor &eri, const std::vector<double> &e, int64_t nOcc, const T1Tensor &t1,
+                               const T2Tensor &t2, const Thread *thread) {
+    const int64_t o = nOcc, n = eri.dimension(0), v = n - o;
+    // the elements of the repack of RCCSD_T.cpp:37-51
+    const auto ovvv = block(o, v, v, v, [&](int64_t i, int64_t a, int64_t b, int64_t f) { return eri(i, o + b, o + a, o + f); });
+    const auto ovoo = block(o, v, o, o, [&](int64_t i, int64_t a, int64_t m, int64_t j) { return eri(i, m, o + a, j); });
+    const auto ovov = block(o, v, o, v, [&](int64_t i, int64_t a, int64_t j, int64_t b) { return eri(i, j, o + a, o + b); });
+    Ctx ctx;
+    const scess_t_in in{o, v, e.data(), t1.data(), t2.data(), ovvv.data(), ovoo.data(), ovov.data()};
+    ctx.check(scess_set_should_stop(ctx.c, stop_cb, (void *)thread));
+    ctx.check(scess_t_set_inputs(ctx.c, &in, 1 << SCESS_PREC_FP64));
+    double E = 0.0;
+    const int rc = scess_t_forward(ctx.c, SCESS_PREC_FP64, &E, nullptr);
+    if (rc == SCESS_CANCELLED)
+        return 0.0;
+    ctx.check(rc);
+    return E;
+}
+
+RCCSD_Wfn RCCSD_T_FromMO(const TEITensor &eri, const std::vector<double> &e, int64_t nOcc, double E_HF,
+                         const Thread *thread, std::stringstream *console) {
+    if (console)
+        *console << "--- Closed-shell CCSD with perturbative triples (CCSD(T)) ---\n";
+    return run_ccsd_t(eri, e, nOcc, E_HF, thread, console);
+}
+
+RCCSD_Wfn RCCSD_T(const HF::HF_Wfn &rhfWfn, const Integrals::MolInts &mints, const Thread *thread,
+                  std::stringstream *console) {
+    int64_t nOcc = 0;  // position of the first unoccupied orbital (RCCSD_T.cpp:172)
+    while (nOcc < (int64_t)rhfWfn.Occ_Alpha.size() && rhfWfn.Occ_Alpha[nOcc] != 0)
+        nOcc++;
+    const TEITensor eri = Integrals::TransformTEI(mints.eri, rhfWfn.C_Alpha, rhfWfn.nmo, true);
+    if (console)
+        *console << "--- Closed-shell CCSD with perturbative triples (CCSD(T)) ---\n";
+    return run_ccsd_t(eri, rhfWfn.Ep_Alpha, nOcc, rhfWfn.E_HF, thread, console);
+}
+
+}  // namespace SCESS::CC
+
+namespace SCESS::Integrals {
+
+TEITensor TransformTEI(const TEITensor &tei, const std::vector<double> &C, int64_t nmo, bool convertChemToPhys) {
+    const int64_t nao = tei.dimension(0);
+    if ((int64_t)C.size() != nao * nmo)
+        throw std::runtime_error("SCESS: TransformTEI: C is not nao x nmo");
+    TEITensor out(nmo, nmo, nmo, nmo);
+    scess_ctx *c = nullptr;
+    if (scess_create(&c, 0) != SCESS_OK)
+        throw std::runtime_error("SCESS: no usable GPU (scess_create failed)");
+    const int rc = scess_transform_tei(c, nao, nmo, tei.data(), C.data(), convertChemToPhys ? 1 : 0, out.data());
+    const std::string err = rc == SCESS_OK ? "" : scess_last_error(c);
+    scess_destroy(c);
+    if (rc != SCESS_OK)
+        throw std::runtime_error("SCESS: " + err);
+    return out;
+}
+
+}  // namespace SCESS::Integrals
+
+namespace SCESS::CC {
+
+RCCSD_Wfn run_ccsd_t(const TEITensor &eri, const std::vector<double> &e, int64_t nOcc, double E_HF,
+                     const Thread *thread, std::stringstream *console) {
+    RCCSD_Wfn wfn = SolveRCCSD_DIIS(eri, e, nOcc, nullptr, thread, console);
+    if (!wfn.converged || (thread && thread->ShouldStop())) {
+        if (console)
+            *console << "Cannot calculate the triples correction as the CCSD amplitudes did not converge.\n";
+        return wfn;
+    }
+    if (console)
+        *console << "Calculating the triples correction to the energy...\n";
+    wfn.E_T = RCCSD_TriplesCorrection(eri, e, nOcc, wfn.t1, wfn.t2, thread);
+    if (console)
+        *console << std::setprecision(12) << "E(T) = " << wfn.E_T << " hartrees\n"
+                 << "E(Total) = E(Hartree Fock) + E(CCSD) + E((T)) = " << E_HF + wfn.E_CCSD + wfn.E_T
+                 << " hartrees\n\n\n";
+    return wfn;
+}
+
+}  // namespace SCESS::CC

@@ -968,6 +968,40 @@ int scess_ccsd_energy(scess_ctx *c, const double *t1, const double *t2, double *
     return SCESS_OK;
 }
 
+int scess_transform_tei(scess_ctx *c, int64_t nao, int64_t nmo, const double *tei, const double *C,
+                        int chem_to_phys, double *out) {
+    if (!c)
+        return SCESS_ERR_ARG;
+    if (!tei || !C || !out || nao <= 0 || nmo <= 0)
+        return fail(c, SCESS_ERR_ARG, "scess_transform_tei: bad argument");
+    HIP_TRY(c, hipSetDevice(c->device));
+    try {
+        scess::Engine &E = c->eng;
+        // one index at a time, as TransformTEI (IntegralTransforms.cpp:33-58), last index first
+        scess::DTensor cm = upload_colmajor(E, C, {nao, nmo});
+        scess::DTensor g = upload_colmajor(E, tei, {nao, nao, nao, nao});
+        scess::DTensor h = E.alloc({nao, nao, nao, nmo});
+        E.einsum("abcd,ds->abcs", g, &cm, h, 1.0, 0.0);
+        g = E.alloc({nao, nao, nmo, nmo});
+        E.einsum("abcs,cr->abrs", h, &cm, g, 1.0, 0.0);
+        h = E.alloc({nao, nmo, nmo, nmo});
+        E.einsum("abrs,bq->aqrs", g, &cm, h, 1.0, 0.0);
+        g = E.alloc({nmo, nmo, nmo, nmo});
+        E.einsum("aqrs,ap->pqrs", h, &cm, g, 1.0, 0.0);
+        if (chem_to_phys) {
+            h = E.alloc({nmo, nmo, nmo, nmo});
+            E.permute("pqrs", g.p, g.dims, "prqs", h.p, h.dims, 1.0, 0.0);
+            download_colmajor(E, h, out);
+        } else {
+            download_colmajor(E, g, out);
+        }
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    } catch (const std::exception &ex) {
+        return engine_fail(c, ex);
+    }
+    return SCESS_OK;
+}
+
 int scess_ccsd_solve(scess_ctx *c, double *t1_out, double *t2_out, scess_ccsd_result *res, scess_ccsd_log log,
                      void *user) {
     if (!c)

@@ -117,6 +117,8 @@ def load(path: str = LIB_PATH):
     lib.scess_ccsd_energy.restype = c.c_int
     lib.scess_ccsd_solve.argtypes = [c.c_void_p, _dp, _dp, c.POINTER(CcsdResult), LOG_FN, c.c_void_p]
     lib.scess_ccsd_solve.restype = c.c_int
+    lib.scess_transform_tei.argtypes = [c.c_void_p, c.c_int64, c.c_int64, _dp, _dp, c.c_int, _dp]
+    lib.scess_transform_tei.restype = c.c_int
     _LIB = lib
     return lib
 
@@ -266,3 +268,15 @@ class Context:
         cb = LOG_FN(lambda _u, it, E, dE, rms: log(it, E, dE, rms)) if log else LOG_FN(0)
         self.check(self.lib.scess_ccsd_solve(self.h, _ptr(t1), _ptr(t2), ctypes.byref(res), cb, None))
         return {k: getattr(res, k) for k, _ in res._fields_} | {"t1": t1, "t2": t2}
+
+    def transform_tei(self, tei, C, chem_to_phys=False):
+        """TransformTEI (IntegralTransforms.cpp:29-59) on the GPU: AO (mu nu|kap lam) -> MO, optionally <pq|rs>."""
+        tei = _f64(tei)
+        C = _f64(C)
+        nao, nmo = C.shape
+        if tei.shape != (nao,) * 4:
+            raise ValueError("tei must be nao^4")
+        out = np.zeros((nmo,) * 4, order="F")
+        self.check(self.lib.scess_transform_tei(self.h, nao, nmo, _ptr(tei), _ptr(C), int(bool(chem_to_phys)),
+                                                _ptr(out)))
+        return out

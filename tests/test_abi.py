@@ -35,3 +35,15 @@ def test_create_without_gpu_returns_error_code():
     h = ctypes.c_void_p()
     assert lib.scess_create(ctypes.byref(h), 0) != 0
     assert lib.scess_last_error(None) == b"null context"
+
+
+def test_library_exports_the_cxx_method_api():
+    """include/scess_cc.h: the C++ mirror of SCESS::CC (CC.h:136-164) is defined in the same library."""
+    import shutil
+    import subprocess
+    if not os.path.exists(gpu.LIB_PATH) or not shutil.which("nm"):
+        pytest.skip("libscess_gpu.so not built or nm missing")
+    out = subprocess.run(["nm", "-DC", "--defined-only", gpu.LIB_PATH], capture_output=True, text=True).stdout
+    for n in ("SCESS::CC::SolveRCCSD_DIIS(", "SCESS::CC::RCCSD_TriplesCorrection(", "SCESS::CC::RCCSD_T(",
+              "SCESS::CC::RCCSD_T_FromMO(", "SCESS::Integrals::TransformTEI("):
+        assert n in out, n

@@ -121,3 +121,18 @@ def test_water_end_to_end_on_gpu(ctx, basis):
     ET, _ = ctx.triples_forward("fp64")
     assert abs(ET - ref[1]) <= ref[3]
     assert abs(ET - oracle.t_abc(inp)) < 1e-10
+
+
+@pytest.mark.parametrize("nao,nmo,phys", [(1, 1, False), (9, 7, False), (9, 9, True), (24, 17, True)])
+def test_transform_tei(ctx, nao, nmo, phys):
+    """TransformTEI on the GPU against numpy's four-index transform (FP64, 1e-12 relative).
+    A transposed C or a missing chemists'->physicists' swap fails it: the tensor has no symmetry."""
+    _say(f"case transform nao={nao} nmo={nmo} phys={phys}")
+    rng = np.random.default_rng(nao * 100 + nmo)
+    tei = rng.standard_normal((nao,) * 4)
+    C = rng.standard_normal((nao, nmo))
+    ref = np.einsum("abcd,ap,bq,cr,ds->pqrs", tei, C, C, C, C, optimize=True)
+    if phys:
+        ref = ref.transpose(0, 2, 1, 3)
+    out = ctx.transform_tei(tei, C, chem_to_phys=phys)
+    assert np.abs(out - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max())
