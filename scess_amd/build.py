@@ -14,6 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libscess_gpu.so")
+CLI = os.path.join(HERE, "scess_cc")
 SOURCES = ["triples_kernels.hip", "triples_adjoint.hip", "ccsd_engine.hip", "scess_capi.hip", "cc_api.cpp"]
 ARCH = "gfx950"
 
@@ -53,6 +54,10 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,--no-undefined", "-o", tmp, *objs])
     os.replace(tmp, LIB)
+    # command-line runner over the C++ method API (host code only)
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-o", CLI + ".tmp", os.path.join(CSRC, "scess_cc_main.cpp"),
+          "-L", HERE, "-lscess_gpu", "-Wl,-rpath,$ORIGIN"])
+    os.replace(CLI + ".tmp", CLI)
     return LIB
 
 

@@ -136,3 +136,52 @@ def test_transform_tei(ctx, nao, nmo, phys):
         ref = ref.transpose(0, 2, 1, 3)
     out = ctx.transform_tei(tei, C, chem_to_phys=phys)
     assert np.abs(out - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max())
+
+
+def _write_fcidump(path, h, eri, nelec, core):
+    """FCIDUMP of MO integrals: h (n x n), chemists' (pq|rs) (C order), each unique element once."""
+    n = h.shape[0]
+    with open(path, "w") as f:
+        f.write(f" &FCI NORB={n},NELEC={nelec},MS2=0,\n  ORBSYM={','.join(['1'] * n)},\n  ISYM=1,\n &END\n")
+        for p in range(n):
+            for q in range(p + 1):
+                for r in range(n):
+                    for s in range(r + 1):
+                        if p * (p + 1) // 2 + q >= r * (r + 1) // 2 + s and eri[p, q, r, s] != 0.0:
+                            f.write(f"{eri[p, q, r, s]:.17e} {p + 1} {q + 1} {r + 1} {s + 1}\n")
+        for p in range(n):
+            for q in range(p + 1):
+                f.write(f"{h[p, q]:.17e} {p + 1} {q + 1} 0 0\n")
+        f.write(f"{core:.17e} 0 0 0 0\n")
+
+
+@pytest.mark.parametrize("basis", ["STO-3G", "cc-pVDZ"])
+def test_cli_ccsd_t_water(tmp_path, basis):
+    """scess_cc ccsd_t on a water FCIDUMP: the C++ method API (SolveRCCSD_DIIS, RCCSD_TriplesCorrection)
+    end to end on the GPU, against the published energies (Tests.cpp:43-44, 52-53)."""
+    import os
+    import re
+    import subprocess
+    from oracle import scf
+    _say(f"case cli water {basis}")
+    cli = os.path.join(os.path.dirname(gpu.LIB_PATH), "scess_cc")
+    ref = {"STO-3G": (-74.9629832288, -0.0494026324, -0.0000673499, 1e-6),
+           "cc-pVDZ": (-76.0267859418, -0.2133077024, -0.0030574132, 1e-8)}[basis]
+    atoms = scf.water_builtin()
+    S, H, G = scf.ao_integrals(basis, atoms)
+    E_HF, e, C = scf.rhf(S, H, G, 5, scf.nuclear_repulsion(atoms))
+    eri = scf.mo_eri(G, C)
+    dump = tmp_path / "FCIDUMP"
+    _write_fcidump(dump, C.T @ H @ C, eri, 10, scf.nuclear_repulsion(atoms))
+    r = subprocess.run([cli, "ccsd_t", str(dump)], capture_output=True, text=True, timeout=120)
+    _say(r.stdout[-600:], r.stderr[-600:])
+    assert r.returncode == 0
+    num = r"(-?[0-9.]+(?:e[-+]?\d+)?)"
+    ehf = float(re.search(r"E\(Hartree Fock\) = " + num, r.stdout).group(1))
+    ecc = float(re.search(r"E\(CCSD correlation\) = " + num, r.stdout).group(1))
+    et = float(re.search(r"E\(T\) = " + num, r.stdout).group(1))
+    assert abs(ehf - E_HF) < 1e-9
+    assert abs(ehf - ref[0]) < ref[3]
+    assert abs(ecc - ref[1]) < 1e-7
+    assert abs(et - ref[2]) < 1e-7
+    assert "Turning on DIIS" in r.stdout and "CCSD procedure converged" in r.stdout
