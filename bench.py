@@ -93,6 +93,84 @@ def load_traffic(kernel_tag: str):
     return None
 
 
+def adjoint_bench(args):
+    """Backward of E(T) at o=20, v=200 (unless --o/--v are given): time per backward call and a
+    central-difference check (h = 1e-5) of dE/dT2, dE/d(vvvo), dE/d(ooov) on seeded random entries.
+
+    Each difference is local: the forward returns per-(ijk, orbit-run) partials and the partials an
+    entry does not reach are bitwise equal in E(x+h) and E(x-h), so only the touched ones are summed.
+    E is quadratic in every single entry, so the central difference is exact up to rounding."""
+    import numpy as np
+
+    from scess_amd import gpu
+    from scess_amd.synthetic import TriplesInputs, make_triples_inputs
+
+    o = args.o if args.o != 30 else 20
+    v = args.v if args.v != 300 else 200
+    inp = make_triples_inputs(o, v, args.seed)
+    ctx = gpu.Context(0)
+    ctx.set_triples_inputs(inp, ("fp64",))
+    E, st = ctx.triples_forward("fp64")
+    log(f"[adjoint] o={o} v={v} E(T)={E!r} forward {st['seconds']:.3f} s")
+    for w in range(args.warmup):
+        ctx.triples_backward("fp64")
+    times = []
+    for s_ in range(args.steps):
+        t = time.perf_counter()
+        g = ctx.triples_backward("fp64")
+        times.append(time.perf_counter() - t)
+        log(f"[adjoint] backward step {s_}: {times[-1]:.3f} s")
+    sec = sum(times) / len(times)
+
+    def replace(**kw):
+        d = dict(o=inp.o, v=inp.v, e=inp.e, t1=inp.t1, t2=inp.t2, ovvv=inp.ovvv, ovoo=inp.ovoo, ovov=inp.ovov)
+        d.update(kw)
+        return TriplesInputs(**d)
+
+    from scess_amd.triples import ijk_weights
+    wgt = ijk_weights(o)
+    rng = np.random.default_rng(args.seed + 1000)
+    h = 1e-5
+    checks, worst = [], 0.0
+    gmax = {n: float(np.abs(g[n]).max()) for n in ("t2", "ovvv", "ovoo")}
+    for n_ in range(args.fd_entries):
+        name = ("t2", "ovvv", "ovoo")[int(rng.integers(3))]
+        arr = getattr(inp, name)
+        flat = int(rng.integers(arr.size))
+        ix = np.unravel_index(flat, arr.shape, order="F")
+        parts = []
+        for sgn in (1.0, -1.0):
+            a = np.array(arr, order="F", copy=True)
+            a[ix] += sgn * h
+            ctx.set_triples_inputs(replace(**{name: a}), ("fp64",))
+            _, _, P, _ = ctx.triples_forward("fp64", with_F=True, with_partials=True)
+            parts.append(P)
+        diff = (parts[0] - parts[1]) * wgt[:, None]
+        fd = float(np.sum(diff[diff != 0.0])) / (2 * h)
+        an = float(g[name][ix])
+        rel = abs(fd - an) / max(abs(an), 1e-300)
+        worst = max(worst, rel)
+        checks.append({"array": name, "index": [int(x) for x in ix], "analytic": an, "fd": fd, "rel_err": rel,
+                       "abs_over_max": abs(fd - an) / gmax[name]})
+        log(f"[adjoint] fd {n_}: d{name}{tuple(int(x) for x in ix)} analytic {an:.12e} fd {fd:.12e} rel {rel:.2e}")
+    ctx.close()
+    fwd_flops = 2.0 * o ** 3 * v ** 3 * (o + v)
+    line = {
+        "metric": "(T) adjoint seconds per backward (FP64) and central-difference check",
+        "value": sec, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": sec * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"(T) backward dE/d(vvvo, ooov, ovov, T1, T2), o={o}, v={v}, seed {args.seed}",
+                   "o": o, "v": v},
+        "E_T": E,
+        "backward_tflops_vs_3x_forward_flops": 3.0 * fwd_flops / sec / 1e12,
+        "fd": {"h": h, "entries": len(checks), "max_rel_err": worst,
+               "max_abs_err_over_max_grad": max(c["abs_over_max"] for c in checks) if checks else None,
+               "n_rel_err_le_1e-6": sum(c["rel_err"] <= 1e-6 for c in checks), "checks": checks},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,16 +182,22 @@ def main():
     ap.add_argument("--modes", default="fp64,fp32,bf16x3")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--adjoint", action="store_true",
+                    help="time the (T) backward and check it by central differences (default o=20, v=200)")
+    ap.add_argument("--fd-entries", type=int, default=64)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU only; multi-GPU sharding of (T) is a design "
-              "(see DESIGN.md), not measured. Run with --gpus 1.", file=sys.stderr)
+        print("bench.py: this build runs on one GPU only; sharding (T) over several GPUs is "
+              "designed but not built or measured. Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     from scess_amd import gpu
     from scess_amd.synthetic import make_triples_inputs
 
+    if args.adjoint:
+        adjoint_bench(args)
+        return
     o, v = args.o, args.v
     modes = [m for m in args.modes.split(",") if m]
     t0 = time.perf_counter()

@@ -358,7 +358,11 @@ static void plus_transpose_add(Engine &E, DTensor &R, const DTensor &X, double a
 
 // Particle-particle ladder sum_cd (ac|bd) tau_ij^cd from the Cholesky factor, (a,b) tiles of TAB x TAB
 void CcsdState::ladder_from_factor(Engine &E, const DTensor &tau, DTensor &R) {
-    const int64_t TAB = 16;
+    // (a, b) tile edge: 64 fills the GEMM's 64-row tiles; smaller when one tile of integrals
+    // (TAB^2 v^2 doubles) would pass 8 GiB
+    int64_t TAB = 64;
+    while (TAB > 16 && TAB * TAB * v * v * 8 > ((int64_t)8 << 30))
+        TAB /= 2;
     const int64_t nP = naux;
     DTensor I = E.alloc({TAB, v, TAB, v});  // I[a][c][b][d] of one tile
     for (int64_t a0 = 0; a0 < v; a0 += TAB) {

@@ -20,6 +20,20 @@ from . import gpu
 from .synthetic import TriplesInputs
 
 
+def unique_ijk(o: int) -> np.ndarray:
+    """Occupied triples i >= j >= k in the order of the per-ijk outputs of scess_t_forward_ex
+    (i outer, then j <= i, then k <= j), shape (o(o+1)(o+2)/6, 3)."""
+    out = [(i, j, k) for i in range(o) for j in range(i + 1) for k in range(j + 1)]
+    return np.array(out, dtype=np.int32).reshape(-1, 3)
+
+
+def ijk_weights(o: int) -> np.ndarray:
+    """E(T) = sum_ijk w_ijk F_ijk over unique_ijk(o): 2/deg, i.e. 2 for distinct indices, 1 when two
+    are equal, 0 when all three are (r3 annihilates those)."""
+    t = unique_ijk(o)
+    return np.where(t[:, 0] == t[:, 2], 0.0, np.where((t[:, 0] == t[:, 1]) | (t[:, 1] == t[:, 2]), 1.0, 2.0))
+
+
 def blocks_from_eri(eri: np.ndarray, nocc: int):
     """(ia|bf), (ia|mj), (ia|jb) sliced from <pq|rs> = (pr|qs), occupied first.
 

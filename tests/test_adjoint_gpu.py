@@ -63,7 +63,8 @@ def test_backward_scales_with_dE_and_is_deterministic(ctx):
     g3 = ctx.triples_backward("fp64", dE=-2.5)
     for k in NAMES:
         assert np.array_equal(g1[k], g2[k]), k
-        assert np.allclose(g3[k], -2.5 * g1[k], rtol=1e-14, atol=0), k
+        # dE enters the per-ijk weights before the back-contractions, so the rounding differs
+        assert np.abs(g3[k] + 2.5 * g1[k]).max() <= 1e-13 * np.abs(g1[k]).max(), k
 
 
 def local_fd(ctx, inp, name, flat_index, h, precision="fp64"):
@@ -83,6 +84,13 @@ def local_fd(ctx, inp, name, flat_index, h, precision="fp64"):
     return float(np.sum(diff[diff != 0.0])) / (2 * h)
 
 
+# h per input: 1e-5 for T2, (vvvo) and (ooov). E is linear in (ovov) and quadratic in T1, and
+# their derivatives are ~1e-4 of the others (V carries a factor t1 ~ 1e-3), so at h = 1e-5 the
+# rounding of the ~|E|/n partials (~1e-16 |E| / h) is ~1e-6 of those derivatives; the central
+# difference of a quadratic is exact for every h, so h = 1e-3 is used there.
+FD_H = {"ovvv": 1e-5, "ovoo": 1e-5, "t2": 1e-5, "ovov": 1e-3, "t1": 1e-3}
+
+
 @pytest.mark.parametrize("o,v,seed", [(5, 24, 6), (6, 40, 7)])
 def test_backward_against_local_central_differences(ctx, o, v, seed):
     inp = make_triples_inputs(o, v, seed)
@@ -93,7 +101,7 @@ def test_backward_against_local_central_differences(ctx, o, v, seed):
         gmax = np.abs(g[name]).max()
         for f in rng.choice(getattr(inp, name).size, size=4, replace=False):
             _say(f"case fd o={o} v={v} {name}[{f}]")
-            fd = local_fd(ctx, inp, name, int(f), 1e-5)
+            fd = local_fd(ctx, inp, name, int(f), FD_H[name])
             ix = np.unravel_index(int(f), getattr(inp, name).shape, order="F")
             ref = g[name][ix]
             assert abs(fd - ref) <= 1e-6 * max(abs(ref), 1e-3 * gmax), (name, ix, fd, ref)

@@ -142,3 +142,13 @@ def test_backward_against_central_differences(name):
         Em = oracle.t_ijk(_replace(inp, **{name: minus}))[0]
         fd = (Ep - Em) / (2 * h)
         assert abs(fd - g[ix]) <= 1e-7 * max(1.0, abs(g[ix])), (name, ix, fd, g[ix])
+
+
+def test_product_ijk_enumeration_matches_oracle():
+    """scess_amd.triples.unique_ijk / ijk_weights (used to weight per-ijk outputs) follow the oracle's order."""
+    from scess_amd.triples import ijk_weights, unique_ijk
+    for o in (1, 2, 5, 9):
+        t = oracle.unique_ijk(o)
+        assert np.array_equal(unique_ijk(o), t)
+        w = ijk_weights(o)
+        assert w.sum() == 2 * (o * (o - 1) * (o - 2) // 6) + 2 * o * (o - 1) // 2  # 2/deg per unique ijk
