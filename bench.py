@@ -86,11 +86,29 @@ def cpu_baseline(inp, seconds_target: float = 15.0):
 
 
 def load_traffic(kernel_tag: str):
-    path = os.path.join(ROOT, "profiles", f"traffic_{kernel_tag}.json")
+    """HBM bytes per launch of a kernel from a committed PMC summary (profiles/traffic_<tag>.md,
+    a line `bytes_per_launch: N`), or None when there is none."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel_tag}.md")
     if os.path.exists(path):
         with open(path) as f:
-            return json.load(f)
+            for line in f:
+                if line.startswith("bytes_per_launch:"):
+                    return {"bytes_per_launch": float(line.split(":", 1)[1])}
     return None
+
+
+def calibrate_copy(mib: int):
+    """A plain device copy of known size (torch's copy kernel, `mib` MiB read and written, three
+    times) so that FETCH_SIZE / WRITE_SIZE counters can be scaled to bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_per_launch": n * 4, "launches": 3}), flush=True)
 
 
 def adjoint_bench(args):
@@ -185,6 +203,8 @@ def main():
     ap.add_argument("--adjoint", action="store_true",
                     help="time the (T) backward and check it by central differences (default o=20, v=200)")
     ap.add_argument("--fd-entries", type=int, default=64)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (scale for HBM byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -195,6 +215,9 @@ def main():
     from scess_amd import gpu
     from scess_amd.synthetic import make_triples_inputs
 
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.adjoint:
         adjoint_bench(args)
         return

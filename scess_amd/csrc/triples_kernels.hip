@@ -425,10 +425,12 @@ __global__ __launch_bounds__(256) void wbuild_kernel(WBuildArgs a) {
         }
 }
 
+// MFMA blocks per wave edge, from timing sweeps at v = 300: 2 (64x64 work-group tiles, two
+// work-groups per CU) for FP64 and FP32; 4 for BF16x3, whose three MFMAs per product need the
+// larger register tile to keep LDS reads per MFMA low.
 int wbuild_tile(int prec, int v) {
-    (void)prec;
-    if (v >= 240)
-        return 5;
+    if (prec == PB3 && v >= 240)
+        return 4;
     if (v > 32)
         return 2;
     return 1;
