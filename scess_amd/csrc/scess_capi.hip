@@ -24,6 +24,7 @@ struct TriplesState {
     double *S = nullptr, *t1 = nullptr, *e = nullptr, *ev = nullptr;  // FP64 epilogue inputs
     std::vector<int> ijk_all;                          // 3 per unique occupied triple
     std::vector<int> work;                             // indices (into ijk_all) with i != k
+    std::vector<int> work_ijk;                         // 3 per work item, as evaluated (j == k -> (j, j, i))
     int *d_ijk_all = nullptr;
     std::vector<int> orbits;
     int *d_orbits = nullptr;
@@ -52,6 +53,13 @@ struct scess_ctx {
 };
 
 namespace {
+
+// Direct-to-LDS W-build kernel unless SCESS_WBUILD_DMA=0 (the register-staged kernel, for A/B timing)
+bool wbuild_dma() {
+    const char *e = std::getenv("SCESS_WBUILD_DMA");
+    return !(e && e[0] == '0');
+}
+
 
 int fail(scess_ctx *c, int code, const std::string &msg) {
     if (c)
@@ -282,12 +290,22 @@ int scess_t_set_inputs(scess_ctx *c, const scess_t_in *in, int prec_mask) {
     HIP_TRY(c, hipMalloc((void **)&t.d_orbits, sizeof(int) * t.orbits.size()));
     HIP_TRY(c, hipMemcpy(t.d_orbits, t.orbits.data(), sizeof(int) * t.orbits.size(), hipMemcpyHostToDevice));
     {
+        // F_ijk is invariant under permutations of (i, j, k), so a triple with j == k is evaluated
+        // as (j, j, i): every degenerate triple then has i == j, the form the halved W-build takes
         std::vector<int> wijk;
         for (int w : t.work) {
-            wijk.push_back(t.ijk_all[3 * w]);
-            wijk.push_back(t.ijk_all[3 * w + 1]);
-            wijk.push_back(t.ijk_all[3 * w + 2]);
+            const int i = t.ijk_all[3 * w], j = t.ijk_all[3 * w + 1], k = t.ijk_all[3 * w + 2];
+            if (j == k && i != j) {
+                wijk.push_back(j);
+                wijk.push_back(j);
+                wijk.push_back(i);
+            } else {
+                wijk.push_back(i);
+                wijk.push_back(j);
+                wijk.push_back(k);
+            }
         }
+        t.work_ijk = wijk;
         const size_t nw = std::max<size_t>(t.work.size(), 1);
         HIP_TRY(c, hipMalloc((void **)&t.d_work_ijk, sizeof(int) * 3 * nw));
         HIP_TRY(c, hipMalloc((void **)&t.d_work_index, sizeof(int) * nw));
@@ -385,6 +403,7 @@ int scess_t_forward_ex(scess_ctx *c, int prec, double *e_t, double *F_ijk, doubl
     wa.W1 = t.W1;
     wa.W2 = t.W2;
     wa.wstride = wstride;
+    wa.halve = 1;
 
     EpiArgs ea;
     ea.W1 = t.W1;
@@ -401,6 +420,7 @@ int scess_t_forward_ex(scess_ctx *c, int prec, double *e_t, double *F_ijk, doubl
     ea.e = t.e;
     ea.ev = t.ev;
     ea.U = t.U;
+    ea.halve = 1;
 
     std::vector<hipEvent_t> ev;
     auto new_event = [&](hipEvent_t *e) -> hipError_t {
@@ -449,7 +469,7 @@ int scess_t_forward_ex(scess_ctx *c, int prec, double *e_t, double *F_ijk, doubl
         double *pbase = t.partials + (partials_out ? (size_t)b0 * t.ngrp : 0);
         ea.partials = pbase;
         (void)hipEventRecord(be.a, c->stream);
-        he = launch_wbuild(prec, blk, wa, nb, c->stream);
+        he = launch_wbuild(prec, blk, wbuild_dma(), wa, nb, c->stream);
         if (he == hipSuccess)
             he = hipEventRecord(be.b, c->stream);
         if (he == hipSuccess)
@@ -513,7 +533,10 @@ int scess_t_forward_ex(scess_ctx *c, int prec, double *e_t, double *F_ijk, doubl
         stats->n_ijk = nwork;
         stats->flops_algorithmic = 2.0 * (double)o * o * o * (double)v * v * v * (double)(o + v);
         const double rows = (double)round_up(v, 16);
-        stats->flops_executed = (double)nwork * 2.0 * (double)v * rows * rows *
+        int64_t ndeg = 0;  // triples with two equal indices build half of W
+        for (int w : t.work)
+            ndeg += t.ijk_all[3 * w] == t.ijk_all[3 * w + 1] || t.ijk_all[3 * w + 1] == t.ijk_all[3 * w + 2];
+        stats->flops_executed = ((double)nwork - 0.5 * (double)ndeg) * 2.0 * (double)v * rows * rows *
                                 (6.0 * (double)op.vK + 6.0 * (double)op.oK);
     }
     destroy_events();
@@ -598,6 +621,7 @@ int scess_t_backward(scess_ctx *c, int prec, double dE, scess_t_grad *grad) {
     wa.W1 = W1;
     wa.W2 = W2;
     wa.wstride = wstride;
+    wa.halve = 0;  // the adjoint needs the full W1 and W2 of every triple
     EpiArgs ea{};
     ea.W1 = W1;
     ea.W2 = W2;
@@ -632,13 +656,12 @@ int scess_t_backward(scess_ctx *c, int prec, double dE, scess_t_grad *grad) {
                 break;
             }
         }
-        const int wi = t.work[(size_t)w];
-        const int occ[3] = {t.ijk_all[3 * wi], t.ijk_all[3 * wi + 1], t.ijk_all[3 * wi + 2]};
+        const int occ[3] = {t.work_ijk[3 * w], t.work_ijk[3 * w + 1], t.work_ijk[3 * w + 2]};
         const double deg = (occ[0] == occ[1] || occ[1] == occ[2]) ? 2.0 : 1.0;
         const int *dijk = t.d_work_ijk + 3 * w;
         wa.ijk = dijk;
         ea.ijk = dijk;
-        he = launch_wbuild(prec, blk, wa, 1, c->stream);
+        he = launch_wbuild(prec, blk, wbuild_dma(), wa, 1, c->stream);
         if (he == hipSuccess)
             he = launch_vpieces(t.S, dijk, (int)o, (int)v, (int)vR, U, 1, c->stream);
         if (he == hipSuccess)

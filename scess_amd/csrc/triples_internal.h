@@ -46,11 +46,15 @@ struct WBuildArgs {
     const int *ijk;    // 3 ints per batch entry
     void *W1, *W2;     // accumulator type, [nbatch] x vR^3 in 8x8x8 blocks
     int64_t wstride;   // elements per batch entry
+    // 1: an entry with i == j builds only H = the family-1 segments {0, 1, 3, 4, 5, 6} into W1
+    // (W = H(a,b,c) + H(b,a,c) for i == j) and its family-2 work-groups do nothing
+    int halve;
 };
 
 // Tile configuration: 2x2 waves, each owning blk x blk MFMA 16x16 blocks.
 int wbuild_tile(int prec, int v);  // returns blk for this precision and size
-hipError_t launch_wbuild(int prec, int blk, const WBuildArgs &a, int nbatch, hipStream_t s);
+// dma: the direct-to-LDS kernel (blk 2 or 4); otherwise the register-staged one (blk 1, 2, 4, 5)
+hipError_t launch_wbuild(int prec, int blk, bool dma, const WBuildArgs &a, int nbatch, hipStream_t s);
 
 struct EpiArgs {
     const void *W1, *W2;  // accumulator type, [nbatch] x vR^3 in 8x8x8 blocks
@@ -65,6 +69,7 @@ struct EpiArgs {
     const double *ev;     // [vR] virtual orbital energies, zero-padded
     int ngrp;             // work-groups per ijk (runs of consecutive orbits)
     double *partials;     // [nbatch][ngrp]
+    int halve;            // entries with i == j read W1 in place of W2 (see WBuildArgs::halve)
 };
 int epilogue_groups(int norb);
 hipError_t launch_epilogue(int prec, const EpiArgs &a, int nbatch, hipStream_t s);

@@ -25,10 +25,12 @@
 #include "triples_internal.h"
 
 #include <cstdint>
+#include <cstdlib>
 
 namespace scess {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef double d2 __attribute__((ext_vector_type(2)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
@@ -183,6 +185,10 @@ struct Seg {
     int nch;           // K chunks
 };
 
+// Family-1 segments of H for an occupied triple with i == j: one of each pair of permutation
+// terms exchanged by (i,a) <-> (j,b), so that W(a,b,c) = H(a,b,c) + H(b,a,c).
+__device__ inline int halved_seg(int s) { return s < 2 ? s : s + 1; }  // 0 1 3 4 5 6
+
 // Segment s of family 1 (fixed a = fx, rows b, cols c) or family 2 (fixed b = fx,
 // rows a, cols c) for the occupied triple (i, j, k).
 __device__ inline Seg get_seg(const WBuildArgs &a, bool fam1, int s, int64_t i, int64_t j, int64_t k, int64_t fx) {
@@ -267,8 +273,11 @@ __global__ __launch_bounds__(256) void wbuild_kernel(WBuildArgs a) {
     const int v = a.v;
     const int r0 = tr * BM, c0 = tc * BM;
 
-    const int nseg = fam1 ? 8 : 4;
-    const int total = fam1 ? 4 * (a.nchv + a.ncho) : 2 * (a.nchv + a.ncho);
+    const bool deg = a.halve && i == j;
+    if (deg && !fam1)
+        return;  // the whole work-group: W2 of this entry is not used
+    const int nseg = fam1 ? (deg ? 6 : 8) : 4;
+    const int total = fam1 ? (deg ? 3 : 4) * (a.nchv + a.ncho) : 2 * (a.nchv + a.ncho);
 
     // which MFMA blocks of this wave hold rows / columns below v
     bool vr[BLK], vc[BLK];
@@ -305,7 +314,7 @@ __global__ __launch_bounds__(256) void wbuild_kernel(WBuildArgs a) {
         if (++kc == sg.nch) {
             kc = 0;
             if (++s < nseg)
-                sg = get_seg(a, fam1, s, i, j, k, fx);
+                sg = get_seg(a, fam1, deg ? halved_seg(s) : s, i, j, k, fx);
         }
     };
     auto store = [&](int buf, const uint4 *ra, const uint4 *rb) {
@@ -425,9 +434,216 @@ __global__ __launch_bounds__(256) void wbuild_kernel(WBuildArgs a) {
         }
 }
 
-// MFMA blocks per wave edge, from timing sweeps at v = 300: 2 (64x64 work-group tiles, two
-// work-groups per CU) for FP64 and FP32; 4 for BF16x3, whose three MFMAs per product need the
-// larger register tile to keep LDS reads per MFMA low.
+// ------------------------------------------------ W build, direct-to-LDS loads
+// Same tiles, segments and MFMA core as wbuild_kernel, but the operand chunks go
+// from global memory straight into LDS (global_load_lds_dwordx4, 16 bytes per
+// lane, one contiguous KiB per wave instruction) through a ring of STAGES
+// buffers, so no registers hold staged data and no ds_write is issued. Rows are
+// stored unpadded; the 16-byte vectors of row r are permuted by swz(r) so that
+// the MFMA fragment reads (16 rows x 16 bytes per pass) hit distinct banks.
+typedef __attribute__((address_space(3))) void lds_void;
+
+template <int VPR> __device__ inline int lds_swz(int r) {
+    if constexpr (VPR == 8)
+        return (r ^ (r >> 3)) & 7;
+    else
+        return ((r >> 1) ^ (r >> 3)) & 3;
+}
+
+// s_waitcnt vmcnt(N) leaving the other counters alone (gfx9 encoding)
+template <int N> __device__ inline void wait_vm() {
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+template <int PREC, int BLK, int STAGES>
+__global__ __launch_bounds__(256) void wbuild_dma_kernel(WBuildArgs a) {
+    using AccV = typename Tr<PREC>::AccV;
+    using Acc = typename Tr<PREC>::Acc;
+    constexpr int RB = Tr<PREC>::RB;
+    constexpr int VPR = RB / 16;
+    constexpr int BM = 2 * BLK * 16;
+    constexpr int OPB = BM * RB;      // bytes of one operand chunk
+    constexpr int NIW = OPB / 4096;   // wave instructions per operand per chunk (four waves, 1 KiB each)
+    static_assert(OPB % 4096 == 0, "operand chunk must split into whole wave loads");
+    static_assert(STAGES >= 2 && STAGES <= 4, "stages");
+    __shared__ __attribute__((aligned(1024))) char lds[STAGES][2 * OPB];
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = wave >> 1, wn = wave & 1;
+    const int lr = lane & 15, lq = lane >> 4;
+
+    const int nt2 = a.nt * a.nt;
+    const int gx = gridDim.x;
+    const int lin = xcd_remap(blockIdx.y * gx + blockIdx.x, gx * gridDim.y);
+    const int bat = lin / gx;
+    int bid = lin - bat * gx;
+    const bool fam1 = bid < a.ntile1;
+    if (!fam1)
+        bid -= a.ntile1;
+    const int64_t fx = bid / nt2;
+    const int tt = bid - (int)fx * nt2;
+    const int tr = tt / a.nt, tc = tt - tr * a.nt;
+    const int *q = a.ijk + 3 * bat;
+    const int64_t i = q[0], j = q[1], k = q[2];
+    const int v = a.v;
+    const int r0 = tr * BM, c0 = tc * BM;
+
+    const bool deg = a.halve && i == j;
+    if (deg && !fam1)
+        return;  // the whole work-group: W2 of this entry is not used
+    const int nseg = fam1 ? (deg ? 6 : 8) : 4;
+    const int total = fam1 ? (deg ? 3 : 4) * (a.nchv + a.ncho) : 2 * (a.nchv + a.ncho);
+
+    bool vr[BLK], vc[BLK];
+#pragma unroll
+    for (int b = 0; b < BLK; b++) {
+        vr[b] = r0 + wm * BLK * 16 + b * 16 < v;
+        vc[b] = c0 + wn * BLK * 16 + b * 16 < v;
+    }
+
+    AccV acc[BLK][BLK];
+#pragma unroll
+    for (int x = 0; x < BLK; x++)
+#pragma unroll
+        for (int y = 0; y < BLK; y++)
+            acc[x][y] = AccV{};
+
+    // per-lane part of the load addresses (fixed for the whole kernel): row in the tile and the
+    // byte offset of the logical vector this lane fetches for LDS unit n * 64 + lane
+    int lrow[NIW], lvoff[NIW];
+#pragma unroll
+    for (int u = 0; u < NIW; u++) {
+        const int qq = (wave * NIW + u) * 64 + lane;
+        const int r = qq / VPR, p = qq - r * VPR;
+        lrow[u] = r;
+        lvoff[u] = (p ^ lds_swz<VPR>(r)) * 16;
+    }
+
+    int s = 0, kc = 0;
+    Seg sg = get_seg(a, fam1, 0, i, j, k, fx);
+    auto issue = [&](int st) {
+        char *base = lds[st];
+#pragma unroll
+        for (int u = 0; u < NIW; u++) {
+            int gr = r0 + lrow[u], gc = c0 + lrow[u];
+            gr = gr < v ? gr : v - 1;  // rows past v load a valid row; their outputs are never stored
+            gc = gc < v ? gc : v - 1;
+            const int64_t off = (int64_t)kc * RB + lvoff[u];
+            __builtin_amdgcn_global_load_lds((const void *)(sg.A + gr * sg.lda + off),
+                                             (lds_void *)(base + (wave * NIW + u) * 1024), 16, 0, 0);
+            __builtin_amdgcn_global_load_lds((const void *)(sg.B + gc * sg.ldb + off),
+                                             (lds_void *)(base + OPB + (wave * NIW + u) * 1024), 16, 0, 0);
+        }
+        if (++kc == sg.nch) {
+            kc = 0;
+            if (++s < nseg)
+                sg = get_seg(a, fam1, deg ? halved_seg(s) : s, i, j, k, fx);
+        }
+    };
+    // fragment address of logical 16-byte vector lv of tile row rr
+    auto frag = [&](const char *op, int rr, int lv) { return op + rr * RB + ((lv ^ lds_swz<VPR>(rr)) * 16); };
+    auto compute = [&](int st) {
+        const char *As = lds[st];
+        const char *Bs = As + OPB;
+        if constexpr (PREC == P64) {
+#pragma unroll
+            for (int h = 0; h < 2; h++) {  // two k-pairs per 16-byte vector: half the fragment registers
+                d2 af[BLK], bf[BLK];
+#pragma unroll
+                for (int b = 0; b < BLK; b++) {
+                    af[b] = *(const d2 *)frag(As, wm * BLK * 16 + b * 16 + lr, 2 * lq + h);
+                    bf[b] = *(const d2 *)frag(Bs, wn * BLK * 16 + b * 16 + lr, 2 * lq + h);
+                }
+#pragma unroll
+                for (int t = 0; t < 2; t++)
+#pragma unroll
+                    for (int x = 0; x < BLK; x++)
+                        if (vr[x])
+#pragma unroll
+                            for (int y = 0; y < BLK; y++)
+                                if (vc[y])
+                                    acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[x][t], bf[y][t], acc[x][y], 0, 0, 0);
+            }
+        } else if constexpr (PREC == P32) {
+            f4 af[BLK], bf[BLK];
+#pragma unroll
+            for (int b = 0; b < BLK; b++) {
+                af[b] = *(const f4 *)frag(As, wm * BLK * 16 + b * 16 + lr, lq);
+                bf[b] = *(const f4 *)frag(Bs, wn * BLK * 16 + b * 16 + lr, lq);
+            }
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+#pragma unroll
+                for (int x = 0; x < BLK; x++)
+                    if (vr[x])
+#pragma unroll
+                        for (int y = 0; y < BLK; y++)
+                            if (vc[y])
+                                acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[x][t], bf[y][t], acc[x][y], 0, 0, 0);
+        } else {
+            bf8 ah[BLK], al[BLK], bh[BLK], bl[BLK];
+#pragma unroll
+            for (int b = 0; b < BLK; b++) {
+                const int ra = wm * BLK * 16 + b * 16 + lr, rb = wn * BLK * 16 + b * 16 + lr;
+                ah[b] = *(const bf8 *)frag(As, ra, 2 * lq);
+                al[b] = *(const bf8 *)frag(As, ra, 2 * lq + 1);
+                bh[b] = *(const bf8 *)frag(Bs, rb, 2 * lq);
+                bl[b] = *(const bf8 *)frag(Bs, rb, 2 * lq + 1);
+            }
+#pragma unroll
+            for (int x = 0; x < BLK; x++)
+                if (vr[x])
+#pragma unroll
+                    for (int y = 0; y < BLK; y++)
+                        if (vc[y]) {
+                            acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[x], bh[y], acc[x][y], 0, 0, 0);
+                            acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bl[y], acc[x][y], 0, 0, 0);
+                            acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bh[y], acc[x][y], 0, 0, 0);
+                        }
+        }
+    };
+
+#pragma unroll
+    for (int st = 0; st < STAGES - 1; st++)
+        if (st < total)
+            issue(st);
+    for (int c = 0; c < total; c++) {
+        // this wave's loads of chunk c have landed once at most `ahead` later chunks are outstanding
+        const int ahead = min(total - 1 - c, STAGES - 2);
+        if (STAGES > 3 && ahead >= 2)
+            wait_vm<(STAGES > 3 ? 2 : 0) * 2 * NIW>();
+        else if (STAGES > 2 && ahead == 1)
+            wait_vm<(STAGES > 2 ? 1 : 0) * 2 * NIW>();
+        else
+            wait_vm<0>();
+        __syncthreads();  // every wave's chunk c is in LDS; stage (c - 1) % STAGES is free again
+        if (c + STAGES - 1 < total)
+            issue((c + STAGES - 1) % STAGES);
+        compute(c % STAGES);
+    }
+
+    Acc *W = (Acc *)(fam1 ? a.W1 : a.W2) + (int64_t)bat * a.wstride;
+    const int nbR = a.vR >> 3;
+    const int64_t fbase = ((int64_t)(fx >> 3) * nbR) * nbR * 512 + (fx & 7) * 64;
+#pragma unroll
+    for (int x = 0; x < BLK; x++)
+#pragma unroll
+        for (int y = 0; y < BLK; y++) {
+            const int col = c0 + wn * BLK * 16 + y * 16 + lr;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = r0 + wm * BLK * 16 + x * 16 + (PREC == P64 ? lq + 4 * r : 4 * lq + r);
+                if (row < v && col < v)
+                    W[fbase + ((int64_t)(row >> 3) * nbR + (col >> 3)) * 512 + (row & 7) * 8 + (col & 7)] = acc[x][y][r];
+            }
+        }
+}
+
+// MFMA blocks per wave edge, from timing sweeps at v = 300: 2 (64x64 work-group tiles) for FP64
+// and FP32; 4 for BF16x3, whose three MFMAs per product need the larger register tile to keep
+// LDS reads per MFMA low.
 int wbuild_tile(int prec, int v) {
     if (prec == PB3 && v >= 240)
         return 4;
@@ -437,8 +653,23 @@ int wbuild_tile(int prec, int v) {
 }
 
 template <int PREC>
-static hipError_t launch_wbuild_p(int blk, const WBuildArgs &a, int nbatch, hipStream_t s) {
+static hipError_t launch_wbuild_p(int blk, bool dma, const WBuildArgs &a, int nbatch, hipStream_t s) {
     const dim3 grid((unsigned)(2 * a.ntile1), (unsigned)nbatch);
+    if (dma && (blk == 2 || blk == 4)) {
+        // two stages: more resident work-groups per CU beat deeper prefetch (sweeps at v = 300)
+        int st = 2;
+        if (const char *e = std::getenv("SCESS_WBUILD_STAGES"))  // tuning override: 2, 3 or 4
+            st = std::atoi(e) >= 2 && std::atoi(e) <= 4 ? std::atoi(e) : st;
+#define SCESS_DMA_CASE(B, S)                                                                     \
+    if (blk == B && st == S) {                                                                   \
+        hipLaunchKernelGGL((wbuild_dma_kernel<PREC, B, S>), grid, dim3(256), 0, s, a);           \
+        return hipGetLastError();                                                                \
+    }
+        SCESS_DMA_CASE(2, 2) SCESS_DMA_CASE(2, 3) SCESS_DMA_CASE(2, 4)
+        SCESS_DMA_CASE(4, 2) SCESS_DMA_CASE(4, 3) SCESS_DMA_CASE(4, 4)
+#undef SCESS_DMA_CASE
+        return hipErrorInvalidValue;
+    }
     switch (blk) {
     case 1: hipLaunchKernelGGL((wbuild_kernel<PREC, 1>), grid, dim3(256), 0, s, a); break;
     case 2: hipLaunchKernelGGL((wbuild_kernel<PREC, 2>), grid, dim3(256), 0, s, a); break;
@@ -449,12 +680,12 @@ static hipError_t launch_wbuild_p(int blk, const WBuildArgs &a, int nbatch, hipS
     return hipGetLastError();
 }
 
-hipError_t launch_wbuild(int prec, int blk, const WBuildArgs &a, int nbatch, hipStream_t s) {
+hipError_t launch_wbuild(int prec, int blk, bool dma, const WBuildArgs &a, int nbatch, hipStream_t s) {
     if (prec == P64)
-        return launch_wbuild_p<P64>(blk, a, nbatch, s);
+        return launch_wbuild_p<P64>(blk, dma, a, nbatch, s);
     if (prec == P32)
-        return launch_wbuild_p<P32>(blk, a, nbatch, s);
-    return launch_wbuild_p<PB3>(blk, a, nbatch, s);
+        return launch_wbuild_p<P32>(blk, dma, a, nbatch, s);
+    return launch_wbuild_p<PB3>(blk, dma, a, nbatch, s);
 }
 
 // ---------------------------------------------------------------- epilogue
@@ -617,7 +848,7 @@ __global__ __launch_bounds__(EPI_THREADS) void epilogue_kernel(EpiArgs a) {
     const double eocc = a.e[i] + a.e[j] + a.e[k];
     const double *U = a.U + (int64_t)bat * 3 * vr;
     const AccT *W1b = (const AccT *)a.W1 + (int64_t)bat * a.wstride;
-    const AccT *W2b = (const AccT *)a.W2 + (int64_t)bat * a.wstride;
+    const AccT *W2b = (const AccT *)(a.halve && i == j ? a.W1 : a.W2) + (int64_t)bat * a.wstride;
     const int ap = (tid >> 3) & 7;
     const int o0 = grp * EPI_RUN, o1 = min(o0 + EPI_RUN, a.norb);
 
