@@ -580,7 +580,11 @@ int scess_t_backward(scess_ctx *c, int prec, double dE, scess_t_grad *grad) {
             (void)hipFree(p);
         bufs.clear();
     };
-    const int PT = 16, PR = 32;  // K splits of dT (K = v^2) and dRn (K = v^2)
+    // K splits of dT (v x v outputs, K = v^2) and dRn (v x o outputs, K = v^2): enough work-groups
+    // (about 1024) that the long K loops overlap their memory latency across the chip
+    const int64_t tilesT = ((v + 63) / 64) * ((v + 63) / 64), tilesR = ((v + 63) / 64) * ((o + 63) / 64);
+    const int PT = (int)std::min<int64_t>(64, std::max<int64_t>(16, 1024 / tilesT));
+    const int PR = (int)std::min<int64_t>(128, std::max<int64_t>(32, 1024 / tilesR));
     double *dX, *dT, *dRn, *dTM, *dS, *dt1, *GW, *GV, *U, *dU, *dtv, *scr, *out;
     void *W1, *W2;
     hipError_t he = hipSuccess;

@@ -187,13 +187,12 @@ hipError_t launch_adjoint_g(int prec, const EpiArgs &a, double wgt, double *GW, 
 // and writes its partial tile to scratch[p]; gemm_reduce_kernel adds them in split order.
 constexpr int GT = 64, GKC = 16, GPITCH = GT + 1;
 
-__device__ inline void gemm_load_tile(const GemmOp &op, int64_t base, int r0, int rext, int k1, int k2c, int K2,
-                                      double *lds, int tid) {
-    // 64 rows x 16 k, 4 elements per thread along the contiguous dimension
-    const bool kinner = op.s2 == 1 || (op.sr != 1 && (op.s2 < op.sr));
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-        int rr, kk;
+// One operand's 64 rows x 16 k of chunk (k1, k2c): 4 elements per thread, consecutive along the
+// operand's contiguous dimension so that the loads coalesce.
+struct TileLoad {
+    bool kinner;
+    __device__ TileLoad(const GemmOp &op) : kinner(op.s2 == 1 || (op.sr != 1 && (op.s2 < op.sr))) {}
+    __device__ void coords(int tid, int u, int &rr, int &kk) const {
         if (kinner) {
             rr = tid >> 2;
             kk = (tid & 3) * 4 + u;
@@ -201,17 +200,34 @@ __device__ inline void gemm_load_tile(const GemmOp &op, int64_t base, int r0, in
             kk = tid >> 4;
             rr = (tid & 15) * 4 + u;
         }
+    }
+};
+
+__device__ inline void gemm_fetch(const GemmOp &op, const TileLoad &tl, int64_t base, int r0, int rext, int k1, int k2c,
+                                  int K2, double *reg, int tid) {
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        int rr, kk;
+        tl.coords(tid, u, rr, kk);
         const int gr = r0 + rr, gk = k2c * GKC + kk;
-        double x = 0.0;
-        if (gr < rext && gk < K2)
-            x = op.base[base + (int64_t)gr * op.sr + (int64_t)k1 * op.s1 + (int64_t)gk * op.s2];
-        lds[kk * GPITCH + rr] = x;
+        reg[u] = gr < rext && gk < K2 ? op.base[base + (int64_t)gr * op.sr + (int64_t)k1 * op.s1 + (int64_t)gk * op.s2]
+                                      : 0.0;
+    }
+}
+
+__device__ inline void gemm_stash(const TileLoad &tl, const double *reg, double *lds, int tid) {
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        int rr, kk;
+        tl.coords(tid, u, rr, kk);
+        lds[kk * GPITCH + rr] = reg[u];
     }
 }
 
 __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
-    __shared__ double As[GKC * GPITCH];
-    __shared__ double Bs[GKC * GPITCH];
+    // two LDS buffers; the global loads of chunk c + 1 are in flight while chunk c is multiplied
+    __shared__ double As[2][GKC * GPITCH];
+    __shared__ double Bs[2][GKC * GPITCH];
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = wave >> 1, wn = wave & 1;
@@ -220,10 +236,14 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
     const int m0 = tm * GT, n0 = tn * GT;
     const int beta = blockIdx.y, split = blockIdx.z;
 
-    // chunk range of this split
-    int64_t total = 0;
-    for (int s = 0; s < g.nseg; s++)
-        total += (int64_t)g.seg[s].K1 * ((g.seg[s].K2 + GKC - 1) / GKC);
+    // flat chunk index over the segments; this split takes a fixed contiguous range of it
+    int64_t nch[2] = {0, 0};
+    int nk2[2] = {1, 1};
+    for (int s = 0; s < g.nseg; s++) {
+        nk2[s] = (g.seg[s].K2 + GKC - 1) / GKC;
+        nch[s] = (int64_t)g.seg[s].K1 * nk2[s];
+    }
+    const int64_t total = nch[0] + nch[1];
     const int64_t c_begin = total * split / g.splits, c_end = total * (split + 1) / g.splits;
 
     d4 acc[2][2];
@@ -233,38 +253,42 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
         for (int y = 0; y < 2; y++)
             acc[x][y] = d4{0.0, 0.0, 0.0, 0.0};
 
-    int64_t cbase = 0;
-    for (int s = 0; s < g.nseg; s++) {
+    double ra[4], rb[4];
+    auto fetch = [&](int64_t ch) {
+        const int s = ch < nch[0] ? 0 : 1;
+        const int64_t cl = s == 0 ? ch : ch - nch[0];
         const GemmSeg &sg = g.seg[s];
-        const int nk2 = (sg.K2 + GKC - 1) / GKC;
-        const int64_t nch = (int64_t)sg.K1 * nk2;
-        const int64_t lo = c_begin > cbase ? c_begin - cbase : 0;
-        const int64_t hi = c_end - cbase < nch ? c_end - cbase : nch;
-        const int64_t abase = (int64_t)beta * sg.A.sb, bbase = (int64_t)beta * sg.B.sb;
-        for (int64_t ch = lo; ch < hi; ch++) {
-            const int k1 = (int)(ch / nk2), k2c = (int)(ch - (int64_t)k1 * nk2);
-            gemm_load_tile(sg.A, abase, m0, g.M, k1, k2c, sg.K2, As, tid);
-            gemm_load_tile(sg.B, bbase, n0, g.N, k1, k2c, sg.K2, Bs, tid);
-            __syncthreads();
+        const int k1 = (int)(cl / nk2[s]), k2c = (int)(cl - (int64_t)k1 * nk2[s]);
+        gemm_fetch(sg.A, TileLoad(sg.A), (int64_t)beta * sg.A.sb, m0, g.M, k1, k2c, sg.K2, ra, tid);
+        gemm_fetch(sg.B, TileLoad(sg.B), (int64_t)beta * sg.B.sb, n0, g.N, k1, k2c, sg.K2, rb, tid);
+    };
+    if (c_begin < c_end)
+        fetch(c_begin);
+    int buf = 0;
+    for (int64_t ch = c_begin; ch < c_end; ch++) {
+        const GemmSeg &sg = g.seg[ch < nch[0] ? 0 : 1];
+        gemm_stash(TileLoad(sg.A), ra, As[buf], tid);
+        gemm_stash(TileLoad(sg.B), rb, Bs[buf], tid);
+        __syncthreads();
+        if (ch + 1 < c_end)
+            fetch(ch + 1);
 #pragma unroll
-            for (int t = 0; t < 4; t++) {
-                const int kk = t * 4 + (lane >> 4);
-                double af[2], bf[2];
+        for (int t = 0; t < 4; t++) {
+            const int kk = t * 4 + (lane >> 4);
+            double af[2], bf[2];
 #pragma unroll
-                for (int x = 0; x < 2; x++)
-                    af[x] = As[kk * GPITCH + wm * 32 + x * 16 + (lane & 15)];
+            for (int x = 0; x < 2; x++)
+                af[x] = As[buf][kk * GPITCH + wm * 32 + x * 16 + (lane & 15)];
+#pragma unroll
+            for (int y = 0; y < 2; y++)
+                bf[y] = Bs[buf][kk * GPITCH + wn * 32 + y * 16 + (lane & 15)];
+#pragma unroll
+            for (int x = 0; x < 2; x++)
 #pragma unroll
                 for (int y = 0; y < 2; y++)
-                    bf[y] = Bs[kk * GPITCH + wn * 32 + y * 16 + (lane & 15)];
-#pragma unroll
-                for (int x = 0; x < 2; x++)
-#pragma unroll
-                    for (int y = 0; y < 2; y++)
-                        acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[x], bf[y], acc[x][y], 0, 0, 0);
-            }
-            __syncthreads();
+                    acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[x], bf[y], acc[x][y], 0, 0, 0);
         }
-        cbase += nch;
+        buf ^= 1;
     }
 
 #pragma unroll
