@@ -189,6 +189,53 @@ def adjoint_bench(args):
     print(json.dumps(line), flush=True)
 
 
+def ccsd_bench(args):
+    """One closed-shell CCSD amplitude update (UpdateAmplitudes, FP64) at o=50, v=500 unless --o/--v
+    are given, with the particle-particle ladder regenerated from the Cholesky factor (no v^4 tensor),
+    starting from the MP2 amplitudes. Each timed call includes the host<->device copies of t1 and t2."""
+    import numpy as np
+
+    from scess_amd import gpu
+    from scess_amd.synthetic import make_ccsd_inputs, t2_from_ovov
+
+    o = args.o if args.o != 30 else 50
+    v = args.v if args.v != 300 else 500
+    t0 = time.perf_counter()
+    ci = make_ccsd_inputs(o, v, args.seed)
+    log(f"[ccsd] generated o={o} v={v} naux={ci.naux} in {time.perf_counter() - t0:.1f} s")
+    ctx = gpu.Context(0)
+    t0 = time.perf_counter()
+    ctx.set_ccsd_inputs(o, v, ci.e, ci.blocks, chol_vv=ci.chol_vv)
+    log(f"[ccsd] inputs resident in {time.perf_counter() - t0:.1f} s")
+    t1 = np.zeros((o, v), order="F")
+    t2 = t2_from_ovov(ci.blocks["ovov"], ci.e, o)
+    E0 = ctx.ccsd_energy(t1, t2)
+    for w in range(args.warmup):
+        ctx.ccsd_update(t1, t2)
+    times = []
+    for s_ in range(args.steps):
+        t = time.perf_counter()
+        n1, n2 = ctx.ccsd_update(t1, t2)
+        times.append(time.perf_counter() - t)
+        log(f"[ccsd] update {s_}: {times[-1]:.3f} s")
+    E1 = ctx.ccsd_energy(n1, n2)
+    ctx.close()
+    sec = sum(times) / len(times)
+    ladder = 2.0 * o * o * float(v) ** 4
+    line = {
+        "metric": "CCSD amplitude update seconds (FP64, Cholesky-regenerated ladder)",
+        "value": sec, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": sec * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"closed-shell CCSD update from MP2 amplitudes, o={o}, v={v}, naux={ci.naux}, "
+                               f"seed {args.seed}", "o": o, "v": v},
+        "E_MP2": E0, "E_after_one_update": E1,
+        "ladder_flops": ladder, "ladder_regeneration_flops": 2.0 * float(v) ** 4 * ci.naux,
+        "tflops_reference_formulation": (ladder + 16.0 * o ** 3 * float(v) ** 3 + 4.0 * o * float(v) ** 4) / sec / 1e12,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -205,6 +252,10 @@ def main():
     ap.add_argument("--fd-entries", type=int, default=64)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (scale for HBM byte counters)")
+    ap.add_argument("--shard", default="0/1", metavar="S/N",
+                    help="time only shard S of N equal ranges of occupied triples (one GPU's share)")
+    ap.add_argument("--ccsd", action="store_true",
+                    help="time one CCSD amplitude update with the Cholesky ladder (default o=50, v=500)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -221,17 +272,21 @@ def main():
     if args.adjoint:
         adjoint_bench(args)
         return
+    if args.ccsd:
+        ccsd_bench(args)
+        return
     o, v = args.o, args.v
     modes = [m for m in args.modes.split(",") if m]
     t0 = time.perf_counter()
     inp = make_triples_inputs(o, v, args.seed)
     log(f"[bench] generated o={o} v={v} in {time.perf_counter() - t0:.1f} s")
     ctx = gpu.Context(0)
+    shard, nshards = (int(x) for x in args.shard.split("/"))
+    ctx.set_shard(shard, nshards)
     t0 = time.perf_counter()
     ctx.set_triples_inputs(inp, modes)
     log(f"[bench] inputs resident on the GPU in {time.perf_counter() - t0:.1f} s (PCIe upload + repack)")
 
-    triples = float(o) ** 3 * float(v) ** 3
     res = {}
     for m in modes:
         for w in range(args.warmup):
@@ -249,6 +304,8 @@ def main():
             log(f"[bench] {m} step {s}: E={E!r} wall {wall[-1]:.3f} s device {dev[-1]:.3f} s "
                 f"(W-build {wb[-1]:.0f} ms, epilogue {ep[-1]:.0f} ms)")
         sec = sum(wall) / len(wall)
+        # (ijk, abc) tuples evaluated: o^3 v^3 for the whole job, the shard's share otherwise
+        triples = st["flops_algorithmic"] / (2.0 * (o + v))
         res[m] = {
             "E_T": E,
             "seconds_per_step": sec,
@@ -312,7 +369,8 @@ def main():
         "dtype": DTYPE[head],
         "data": "synthetic",
         "config": {
-            "workload": f"closed-shell (T), o={o}, v={v}, synthetic generator seed {args.seed}",
+            "workload": f"closed-shell (T), o={o}, v={v}, synthetic generator seed {args.seed}"
+                        + (f", occupied-triple shard {shard} of {nshards}" if nshards > 1 else ""),
             "o": o, "v": v, "headline_mode": head,
             "tflops_algorithmic": h["tflops_algorithmic"],
         },

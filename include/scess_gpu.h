@@ -114,6 +114,11 @@ int scess_t_forward(scess_ctx *ctx, int prec, double *e_t, scess_t_stats *stats)
 int scess_t_forward_ex(scess_ctx *ctx, int prec, double *e_t, double *F_ijk, double *partials,
                        scess_t_stats *stats);
 int64_t scess_t_num_ijk(const scess_ctx *ctx);
+/* Restrict scess_t_forward / scess_t_backward to one of nshards contiguous, equal-count ranges of the
+ * occupied-triple work list (no counterpart in the reference; default 0 of 1 = everything). The
+ * result is that shard's share: the sum over shards of E(T), F_ijk and the gradients is the full
+ * result, which is how the work would be split over several GPUs. */
+int scess_t_set_shard(scess_ctx *ctx, int64_t shard, int64_t nshards);
 int64_t scess_t_num_partials(const scess_ctx *ctx);
 
 /* Adjoint of E(T) (no counterpart in the reference): writes dE * dE(T)/d(input)

@@ -47,6 +47,7 @@ struct scess_ctx {
     int (*should_stop)(void *) = nullptr;
     void *user = nullptr;
     int64_t nbatch_req = 0;
+    int64_t shard = 0, nshards = 1;  // (T) evaluates work items [n*shard/nshards, n*(shard+1)/nshards)
     TriplesState t;
     scess::Engine eng;
     scess::CcsdState cc;
@@ -168,6 +169,16 @@ int scess_set_batch(scess_ctx *c, int64_t nbatch) {
     if (!c || nbatch < 0)
         return SCESS_ERR_ARG;
     c->nbatch_req = nbatch;
+    return SCESS_OK;
+}
+
+int scess_t_set_shard(scess_ctx *c, int64_t shard, int64_t nshards) {
+    if (!c)
+        return SCESS_ERR_ARG;
+    if (nshards < 1 || shard < 0 || shard >= nshards)
+        return fail(c, SCESS_ERR_ARG, "scess_t_set_shard: need 0 <= shard < nshards");
+    c->shard = shard;
+    c->nshards = nshards;
     return SCESS_OK;
 }
 
@@ -342,6 +353,8 @@ int scess_t_forward_ex(scess_ctx *c, int prec, double *e_t, double *F_ijk, doubl
     const int64_t o = t.o, v = t.v, vR = t.vR;
     const int64_t nall = (int64_t)t.ijk_all.size() / 3;
     const int64_t nwork = (int64_t)t.work.size();
+    const int64_t w_begin = nwork * c->shard / c->nshards, w_end = nwork * (c->shard + 1) / c->nshards;
+    const int64_t nshare = w_end - w_begin;
     const size_t accb = prec == P64 ? 8 : 4;
     const int64_t wstride = vR * vR * vR;  // W in 8x8x8 blocks over vR^3
 
@@ -351,7 +364,7 @@ int scess_t_forward_ex(scess_ctx *c, int prec, double *e_t, double *F_ijk, doubl
         const double per = 2.0 * (double)wstride * (double)accb;
         nbatch = (int64_t)std::max(1.0, std::min(16.0, 4.0e9 / per));
     }
-    nbatch = std::max<int64_t>(1, std::min<int64_t>(nbatch, std::max<int64_t>(nwork, 1)));
+    nbatch = std::max<int64_t>(1, std::min<int64_t>(nbatch, std::max<int64_t>(nshare, 1)));
     if (nbatch > 65535)
         nbatch = 65535;
     const size_t wneed = (size_t)nbatch * (size_t)wstride * accb;
@@ -447,7 +460,7 @@ int scess_t_forward_ex(scess_ctx *c, int prec, double *e_t, double *F_ijk, doubl
 
     int rc = SCESS_OK;
     bool cancelled = false;
-    for (int64_t b0 = 0; b0 < nwork; b0 += nbatch) {
+    for (int64_t b0 = w_begin; b0 < w_end; b0 += nbatch) {
         // keep at most two batches in flight so cancellation is polled while the device works
         if (bev.size() >= 2 && hipEventSynchronize(bev[bev.size() - 2].c) != hipSuccess) {
             rc = fail(c, SCESS_ERR_HIP, "hipEventSynchronize");
@@ -457,7 +470,7 @@ int scess_t_forward_ex(scess_ctx *c, int prec, double *e_t, double *F_ijk, doubl
             cancelled = true;
             break;
         }
-        const int nb = (int)std::min<int64_t>(nbatch, nwork - b0);
+        const int nb = (int)std::min<int64_t>(nbatch, w_end - b0);
         wa.ijk = t.d_work_ijk + 3 * b0;
         ea.ijk = t.d_work_ijk + 3 * b0;
         hipError_t he = hipSuccess;
@@ -507,7 +520,7 @@ int scess_t_forward_ex(scess_ctx *c, int prec, double *e_t, double *F_ijk, doubl
             std::memset(partials_out, 0, sizeof(double) * (size_t)nall * t.ngrp);
             std::vector<double> hp((size_t)nwork * t.ngrp);
             he = hipMemcpy(hp.data(), t.partials, sizeof(double) * hp.size(), hipMemcpyDeviceToHost);
-            for (int64_t w = 0; he == hipSuccess && w < nwork; w++)
+            for (int64_t w = w_begin; he == hipSuccess && w < w_end; w++)
                 std::memcpy(partials_out + (size_t)t.work[(size_t)w] * t.ngrp, hp.data() + (size_t)w * t.ngrp,
                             sizeof(double) * t.ngrp);
         }
@@ -530,13 +543,23 @@ int scess_t_forward_ex(scess_ctx *c, int prec, double *e_t, double *F_ijk, doubl
         }
         stats->w_build_launches = (int64_t)bev.size();
         stats->epilogue_launches = (int64_t)bev.size();
-        stats->n_ijk = nwork;
-        stats->flops_algorithmic = 2.0 * (double)o * o * o * (double)v * v * v * (double)(o + v);
-        const double rows = (double)round_up(v, 16);
+        stats->n_ijk = nshare;
+        // the shard's share of the o^3 ordered occupied triples: 6 per distinct, 3 per two-equal
+        double ordered = 0.0, ordered_all = 0.0;
         int64_t ndeg = 0;  // triples with two equal indices build half of W
-        for (int w : t.work)
-            ndeg += t.ijk_all[3 * w] == t.ijk_all[3 * w + 1] || t.ijk_all[3 * w + 1] == t.ijk_all[3 * w + 2];
-        stats->flops_executed = ((double)nwork - 0.5 * (double)ndeg) * 2.0 * (double)v * rows * rows *
+        for (int64_t w = 0; w < nwork; w++) {
+            const int wi = t.work[(size_t)w];
+            const bool deg = t.ijk_all[3 * wi] == t.ijk_all[3 * wi + 1] || t.ijk_all[3 * wi + 1] == t.ijk_all[3 * wi + 2];
+            ordered_all += deg ? 3.0 : 6.0;
+            if (w >= w_begin && w < w_end) {
+                ordered += deg ? 3.0 : 6.0;
+                ndeg += deg;
+            }
+        }
+        stats->flops_algorithmic = 2.0 * (double)o * o * o * (double)v * v * v * (double)(o + v) *
+                                   (ordered_all > 0.0 ? ordered / ordered_all : 1.0);
+        const double rows = (double)round_up(v, 16);
+        stats->flops_executed = ((double)nshare - 0.5 * (double)ndeg) * 2.0 * (double)v * rows * rows *
                                 (6.0 * (double)op.vK + 6.0 * (double)op.oK);
     }
     destroy_events();
@@ -649,7 +672,8 @@ int scess_t_backward(scess_ctx *c, int prec, double dE, scess_t_grad *grad) {
     int rc = SCESS_OK;
     bool cancelled = false;
     const int64_t nwork = (int64_t)t.work.size();
-    for (int64_t w = 0; w < nwork && rc == SCESS_OK; w++) {
+    const int64_t w_begin = nwork * c->shard / c->nshards, w_end = nwork * (c->shard + 1) / c->nshards;
+    for (int64_t w = w_begin; w < w_end && rc == SCESS_OK; w++) {
         if ((w & 7) == 0) {
             if ((he = hipStreamSynchronize(c->stream)) != hipSuccess) {
                 rc = hip_fail(c, he, "scess_t_backward");

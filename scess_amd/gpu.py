@@ -103,6 +103,8 @@ def load(path: str = LIB_PATH):
     lib.scess_t_forward.restype = c.c_int
     lib.scess_t_forward_ex.argtypes = [c.c_void_p, c.c_int, _dp, _dp, _dp, c.POINTER(TStats)]
     lib.scess_t_forward_ex.restype = c.c_int
+    lib.scess_t_set_shard.argtypes = [c.c_void_p, c.c_int64, c.c_int64]
+    lib.scess_t_set_shard.restype = c.c_int
     lib.scess_t_num_ijk.argtypes = [c.c_void_p]
     lib.scess_t_num_ijk.restype = c.c_int64
     lib.scess_t_num_partials.argtypes = [c.c_void_p]
@@ -179,6 +181,10 @@ class Context:
         """fn() -> bool, polled between batches of occupied triples."""
         self._stop = STOP_FN(lambda _u: 1 if fn() else 0) if fn is not None else STOP_FN(0)
         self.check(self.lib.scess_set_should_stop(self.h, self._stop, None))
+
+    def set_shard(self, shard: int, nshards: int):
+        """Evaluate only shard `shard` of `nshards` equal ranges of occupied triples."""
+        self.check(self.lib.scess_t_set_shard(self.h, int(shard), int(nshards)))
 
     def set_triples_inputs(self, inp, precisions=("fp64",)):
         arrs = dict(e=_f64(inp.e), t1=_f64(inp.t1), t2=_f64(inp.t2), ovvv=_f64(inp.ovvv),

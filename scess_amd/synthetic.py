@@ -114,3 +114,38 @@ def full_eri_phys(L: np.ndarray) -> np.ndarray:
     reference's TEITensor (src/Quantum/Integrals/IntegralTransforms.cpp:29-59)."""
     chem = full_eri_chem(L)
     return np.asfortranarray(chem.transpose(0, 2, 1, 3))
+
+
+@dataclasses.dataclass
+class CcsdInputs:
+    """Integral blocks of SolveRCCSD_DIIS (src/Quantum/CC/RCCSD.cpp:31-44) in the C-ABI layouts
+    (ColMajor, chemists' notation) plus the Cholesky factor of (ac|bd); no v^4 tensor."""
+
+    o: int
+    v: int
+    e: np.ndarray
+    blocks: dict  # ovov (k,c,l,d), oovv (k,i,a,c), ovvo (k,c,a,i), ovvv (k,d,a,c), ovoo (l,c,k,i), oooo (k,i,l,j)
+    chol_vv: np.ndarray  # (a, c, P)
+    naux: int
+
+
+def _pair(L: np.ndarray, s1: slice, s2: slice, s3: slice, s4: slice) -> np.ndarray:
+    """(pq|rs) for p in s1, q in s2, r in s3, s in s4 from L[P, p, q], ColMajor (p, q, r, s)."""
+    A = L[:, s1, s2]
+    B = L[:, s3, s4]
+    naux = L.shape[0]
+    # C-order (s, r, q, p) = sum_P B[P, r, s] A[P, p, q]; its transpose is ColMajor (p, q, r, s)
+    Bm = np.ascontiguousarray(B.transpose(2, 1, 0)).reshape(-1, naux)
+    Am = np.ascontiguousarray(A.transpose(0, 2, 1)).reshape(naux, -1)
+    out = Bm @ Am
+    return out.reshape(B.shape[2], B.shape[1], A.shape[2], A.shape[1]).transpose(3, 2, 1, 0)
+
+
+def make_ccsd_inputs(o: int, v: int, seed: int = 1) -> CcsdInputs:
+    """CCSD blocks of the synthetic system (same draws as make_triples_inputs) built from the factor."""
+    e, L, _ = _draw(o, v, seed)
+    O, V = slice(0, o), slice(o, o + v)
+    blocks = {"ovov": _pair(L, O, V, O, V), "oovv": _pair(L, O, O, V, V), "ovvo": _pair(L, O, V, V, O),
+              "ovvv": _pair(L, O, V, V, V), "ovoo": _pair(L, O, V, O, O), "oooo": _pair(L, O, O, O, O)}
+    chol = np.asfortranarray(L[:, o:, o:].transpose(1, 2, 0))
+    return CcsdInputs(o=o, v=v, e=e, blocks=blocks, chol_vv=chol, naux=L.shape[0])

@@ -185,3 +185,28 @@ def test_cli_ccsd_t_water(tmp_path, basis):
     assert abs(ecc - ref[1]) < 1e-7
     assert abs(et - ref[2]) < 1e-7
     assert "Turning on DIIS" in r.stdout and "CCSD procedure converged" in r.stdout
+
+
+def test_water_baseline_geometry_gpu_vs_oracle(ctx):
+    """Water at O-H 0.9572 A, H-O-H 104.52 deg, cc-pVDZ: CCSD and (T) on the GPU against the CPU
+    oracles (1e-10 Eh); no published energy exists for this geometry."""
+    import oracle
+    from oracle import scf
+    from scess_amd.synthetic import TriplesInputs
+    _say("case water baseline geometry cc-pVDZ")
+    r = scf.molecule_inputs("cc-pVDZ", scf.water_baseline_geometry())
+    o, eri, e = r["nocc"], r["eri"], r["e"]
+    v = e.size - o
+    O, V = slice(0, o), slice(o, None)
+    blocks = {"ovov": eri[O, V, O, V], "oovv": eri[O, O, V, V], "ovvo": eri[O, V, V, O], "ovvv": eri[O, V, V, V],
+              "ovoo": eri[O, V, O, O], "oooo": eri[O, O, O, O], "vvvv": eri[V, V, V, V]}
+    ctx.set_ccsd_inputs(o, v, e, blocks)
+    cc = ctx.ccsd_solve()
+    cco = co.solve_rccsd_diis(eri, e, o)
+    assert cc["converged"] and abs(cc["E_CCSD"] - cco["E_CCSD"]) < 1e-10
+    inp = TriplesInputs(o=o, v=v, e=e, t1=cc["t1"], t2=cc["t2"], ovvv=np.asfortranarray(eri[O, V, V, V]),
+                        ovoo=np.asfortranarray(eri[O, V, O, O]), ovov=np.asfortranarray(eri[O, V, O, V]))
+    ctx.set_triples_inputs(inp, ("fp64",))
+    ET, _ = ctx.triples_forward("fp64")
+    assert abs(ET - oracle.t_abc(inp)) < 1e-10
+    _say(f"  E(CCSD)={cc['E_CCSD']!r} E(T)={ET!r}")

@@ -56,3 +56,19 @@ def test_f12_driver_and_coupling_energy():
     r = co.solve_rccsd_diis(eri, e, 2, f12={"residual": res, "energy": en})
     assert r["converged"]
     assert np.isfinite(r["E_CCSD"])
+
+
+def test_synthetic_ccsd_blocks_match_full_eri():
+    """make_ccsd_inputs builds the chemists' blocks and the (ac|bd) factor from L without n^4 work."""
+    from scess_amd.synthetic import _draw, full_eri_chem, make_ccsd_inputs
+    o, v = 3, 7
+    ci = make_ccsd_inputs(o, v, 4)
+    _, L, _ = _draw(o, v, 4)
+    eri = full_eri_chem(L)
+    O, V = slice(0, o), slice(o, None)
+    ref = {"ovov": eri[O, V, O, V], "oovv": eri[O, O, V, V], "ovvo": eri[O, V, V, O], "ovvv": eri[O, V, V, V],
+           "ovoo": eri[O, V, O, O], "oooo": eri[O, O, O, O]}
+    for k, x in ref.items():
+        assert np.allclose(ci.blocks[k], x, rtol=0, atol=1e-14), k
+        assert ci.blocks[k].flags.f_contiguous, k
+    assert np.allclose(np.einsum("acP,bdP->acbd", ci.chol_vv, ci.chol_vv), eri[V, V, V, V], rtol=0, atol=1e-14)

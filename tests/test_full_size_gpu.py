@@ -1,0 +1,56 @@
+"""Parity at the full size of the synthetic FP64 configuration (o=20, v=200), through
+the C ABI: the GPU's per-triple F_ijk against the oracle's ijk-driven restatement on a
+seeded sample of occupied triples (the oracle needs ~10 s per triple here, so the whole
+E(T) is checked through the sample plus size-independent identities):
+
+* F_ijk on the sample, FP64, 1e-12 relative — one triple with distinct indices and one
+  with j == k (evaluated on the GPU as (j, j, i) through the halved W-build);
+* E(T) = sum_ijk (2/deg) F_ijk with the GPU's own F (the reduction), 1e-13 relative;
+* FP32 / BF16x3 per-triple F within 1e-6 / 1e-4 relative of FP64 (their operand precision).
+"""
+
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from scess_amd import gpu
+from scess_amd.synthetic import make_triples_inputs
+from scess_amd.triples import ijk_weights, unique_ijk
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def test_o20_v200_fp64_against_oracle_sample():
+    o, v = 20, 200
+    _say(f"case full-size o={o} v={v}: generating")
+    inp = make_triples_inputs(o, v, 1)
+    ctx = gpu.Context(0)
+    try:
+        ctx.set_triples_inputs(inp, ("fp64", "fp32", "bf16x3"))
+        E, F, st = ctx.triples_forward("fp64", with_F=True)
+        _say(f"  GPU E(T)={E!r} in {st['seconds']:.2f} s")
+        w = ijk_weights(o)
+        assert abs(E - float(np.sum(w * F))) <= 1e-13 * abs(E)
+        t = unique_ijk(o)
+        rng = np.random.default_rng(20200)
+        distinct = np.flatnonzero((t[:, 0] > t[:, 1]) & (t[:, 1] > t[:, 2]))
+        jk_equal = np.flatnonzero((t[:, 0] > t[:, 1]) & (t[:, 1] == t[:, 2]))
+        items = np.array([rng.choice(distinct), rng.choice(jk_equal)], dtype=np.int64)
+        _say(f"  oracle on triples {[tuple(t[i]) for i in items]}")
+        _, Fo = oracle.t_ijk(inp, items=items)
+        for n, it in enumerate(items):
+            assert abs(F[it] - Fo[n]) <= 1e-12 * abs(Fo[n]), (tuple(t[it]), F[it], Fo[n])
+        for mode, tol in (("fp32", 1e-6), ("bf16x3", 1e-4)):
+            Em, Fm, _ = ctx.triples_forward(mode, with_F=True)
+            rel = np.abs(Fm - F)[w > 0] / np.abs(F[w > 0])
+            _say(f"  {mode}: E(T)={Em!r} max per-triple rel {rel.max():.2e}")
+            assert rel.max() <= tol
+    finally:
+        ctx.close()

@@ -118,3 +118,35 @@ def test_rccsd_triplescorrection_full_eri_api():
     eri = full_eri_phys(inp.L)
     E = RCCSD_TriplesCorrection(eri, inp.e, 4, inp.t1, inp.t2)
     assert _rel(E, oracle.t_abc(inp)) < 1e-12
+
+
+def test_shards_partition_the_work(ctx):
+    """scess_t_set_shard: every occupied triple is evaluated by exactly one shard (F bitwise equal
+    to the unsharded run), E(T) and the adjoint add up over shards."""
+    _say("case shards o=6 v=40")
+    inp = make_triples_inputs(6, 40, 21)
+    ctx.set_triples_inputs(inp, ("fp64",))
+    E, F, _ = ctx.triples_forward("fp64", with_F=True)
+    g = ctx.triples_backward("fp64")
+    n = 3
+    Fs = np.zeros_like(F)
+    Es = 0.0
+    gs = {k: np.zeros_like(x) for k, x in g.items()}
+    try:
+        for s in range(n):
+            ctx.set_shard(s, n)
+            Ei, Fi, _ = ctx.triples_forward("fp64", with_F=True)
+            assert np.all((Fi == 0.0) | (Fs == 0.0))  # disjoint
+            Fs += Fi
+            Es += Ei
+            gi = ctx.triples_backward("fp64")
+            for k in gs:
+                gs[k] += gi[k]
+    finally:
+        ctx.set_shard(0, 1)
+    assert np.array_equal(Fs, F)
+    assert _rel(Es, E) < 1e-13
+    for k in g:
+        assert np.abs(gs[k] - g[k]).max() <= 1e-12 * np.abs(g[k]).max(), k
+    with pytest.raises(gpu.ScessError):
+        ctx.set_shard(3, 3)
