@@ -123,8 +123,8 @@ def adjoint_bench(args):
     from scess_amd import gpu
     from scess_amd.synthetic import TriplesInputs, make_triples_inputs
 
-    o = args.o if args.o != 30 else 20
-    v = args.v if args.v != 300 else 200
+    o = args.o if args.o is not None else 20
+    v = args.v if args.v is not None else 200
     inp = make_triples_inputs(o, v, args.seed)
     ctx = gpu.Context(0)
     ctx.set_triples_inputs(inp, ("fp64",))
@@ -198,8 +198,8 @@ def ccsd_bench(args):
     from scess_amd import gpu
     from scess_amd.synthetic import make_ccsd_inputs, t2_from_ovov
 
-    o = args.o if args.o != 30 else 50
-    v = args.v if args.v != 300 else 500
+    o = args.o if args.o is not None else 50
+    v = args.v if args.v is not None else 500
     t0 = time.perf_counter()
     ci = make_ccsd_inputs(o, v, args.seed)
     log(f"[ccsd] generated o={o} v={v} naux={ci.naux} in {time.perf_counter() - t0:.1f} s")
@@ -241,8 +241,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--o", type=int, default=30)
-    ap.add_argument("--v", type=int, default=300)
+    ap.add_argument("--o", type=int, default=None, help="occupied orbitals (default 30; 20 --adjoint; 50 --ccsd)")
+    ap.add_argument("--v", type=int, default=None, help="virtual orbitals (default 300; 200 --adjoint; 500 --ccsd)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--modes", default="fp64,fp32,bf16x3")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -275,7 +275,8 @@ def main():
     if args.ccsd:
         ccsd_bench(args)
         return
-    o, v = args.o, args.v
+    o = args.o if args.o is not None else 30
+    v = args.v if args.v is not None else 300
     modes = [m for m in args.modes.split(",") if m]
     t0 = time.perf_counter()
     inp = make_triples_inputs(o, v, args.seed)

@@ -369,7 +369,7 @@ void CcsdState::ladder_from_factor(Engine &E, const DTensor &tau, DTensor &R) {
         const int64_t ta = std::min(TAB, v - a0);
         for (int64_t b0 = 0; b0 < v; b0 += TAB) {
             const int64_t tb = std::min(TAB, v - b0);
-            // I[(a,c)][(b,d)] = sum_P L[P][a][c] L[P][b][d]
+            // I[(a,c)][(b,d)] = sum_P L[a][c][P] L[b][d][P]
             GemmArgs g{};
             g.nseg = 1;
             g.M = (int)(ta * v);
@@ -378,8 +378,8 @@ void CcsdState::ladder_from_factor(Engine &E, const DTensor &tau, DTensor &R) {
             g.splits = 1;
             g.seg[0].K1 = 1;
             g.seg[0].K2 = (int)nP;
-            g.seg[0].A = {Lvv.p + a0 * v, 1, 0, v * v, 0};
-            g.seg[0].B = {Lvv.p + b0 * v, 1, 0, v * v, 0};
+            g.seg[0].A = {Lvv.p + a0 * v * nP, nP, 0, 1, 0};  // rows (a, c), K = P contiguous
+            g.seg[0].B = {Lvv.p + b0 * v * nP, nP, 0, 1, 0};
             check(hipMemsetAsync(I.p, 0, sizeof(double) * (size_t)I.size(), E.stream), "memset I");
             g.C = I.p;
             g.sCm = tb * v;

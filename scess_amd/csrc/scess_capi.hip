@@ -958,9 +958,8 @@ int scess_ccsd_set_inputs(scess_ctx *c, const scess_ccsd_in *in) {
             E.einsum("acbd->abcd", vvvv, nullptr, S.vvvv_phys, 1.0, 0.0);
         } else {
             S.naux = in->naux;
-            scess::DTensor L = upload_colmajor(E, in->chol_vv, {v, v, in->naux});  // (a,c,P)
-            S.Lvv = E.alloc({in->naux, v, v});
-            E.einsum("acP->Pac", L, nullptr, S.Lvv, 1.0, 0.0);
+            // kept as [a][c][P]: the ladder's tile regeneration contracts over the contiguous P
+            S.Lvv = upload_colmajor(E, in->chol_vv, {v, v, in->naux});
         }
         S.e = E.alloc({o + v});
         E.upload(S.e, in->e);

@@ -16,6 +16,7 @@
 #include "triples_internal.h"
 
 #include <cstdint>
+#include <cstdlib>
 
 namespace scess {
 
@@ -322,9 +323,184 @@ __global__ __launch_bounds__(256) void gemm_reduce_kernel(GemmArgs g) {
     g.C[(int64_t)m * g.sCm + (int64_t)n * g.sCn] += g.alpha * s;
 }
 
+// ------------------------------------------- GEMM with K-contiguous operands
+// The same contraction when every segment has A.s2 == B.s2 == 1 (the inner K index is the
+// contiguous one, as the einsum engine lays its operands out) and even strides: rows are read as
+// 16-byte vectors (two doubles of K), staged through registers into a double-buffered LDS ring
+// with the rows' vectors XOR-permuted (conflict-free 16-lane fragment reads), and each of the 2x2
+// waves holds BLK x BLK MFMA 16x16 blocks. K tails and rows past M / N are zero-filled.
+typedef double d2v __attribute__((ext_vector_type(2)));
+
+__device__ inline int kswz(int r) { return (r ^ (r >> 3)) & 7; }
+
+template <int BLK, bool VEC>
+__global__ __launch_bounds__(256) void gemm_kc_kernel(GemmArgs g) {
+    constexpr int BM = 2 * BLK * 16;   // tile rows (and columns)
+    constexpr int RB = 128;            // bytes of one 16-double K chunk of a row
+    constexpr int NV = BM * 8 / 256;   // 16-byte vectors per thread per operand per chunk
+    __shared__ __attribute__((aligned(16))) char lds[2][2 * BM * RB];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = wave >> 1, wn = wave & 1;
+    const int lr = lane & 15, lq = lane >> 4;
+    const int ntn = (g.N + BM - 1) / BM;
+    const int tm = blockIdx.x / ntn, tn = blockIdx.x - tm * ntn;
+    const int m0 = tm * BM, n0 = tn * BM;
+    const int beta = blockIdx.y, split = blockIdx.z;
+
+    int64_t nch[2] = {0, 0};
+    int nk2[2] = {1, 1};
+    for (int s = 0; s < g.nseg; s++) {
+        nk2[s] = (g.seg[s].K2 + 15) / 16;
+        nch[s] = (int64_t)g.seg[s].K1 * nk2[s];
+    }
+    const int64_t total = nch[0] + nch[1];
+    const int64_t c_begin = total * split / g.splits, c_end = total * (split + 1) / g.splits;
+
+    d4 acc[BLK][BLK];
+#pragma unroll
+    for (int x = 0; x < BLK; x++)
+#pragma unroll
+        for (int y = 0; y < BLK; y++)
+            acc[x][y] = d4{0.0, 0.0, 0.0, 0.0};
+
+    // thread t loads vectors q = t + 256 u (u < NV) of the BM x 8 vector tile: row q / 8, vector q % 8
+    d2v ra[NV], rb[NV];
+    auto fetch = [&](int64_t ch) {
+        const int s = ch < nch[0] ? 0 : 1;
+        const int64_t cl = s == 0 ? ch : ch - nch[0];
+        const GemmSeg &sg = g.seg[s];
+        const int k1 = (int)(cl / nk2[s]), k0 = (int)(cl - (int64_t)k1 * nk2[s]) * 16;
+        const double *pa = sg.A.base + (int64_t)beta * sg.A.sb + (int64_t)k1 * sg.A.s1;
+        const double *pb = sg.B.base + (int64_t)beta * sg.B.sb + (int64_t)k1 * sg.B.s1;
+#pragma unroll
+        for (int u = 0; u < NV; u++) {
+            const int q = tid + 256 * u, row = q >> 3, k = k0 + 2 * (q & 7);
+            const int gm = m0 + row, gn = n0 + row;
+            d2v a = {0.0, 0.0}, b = {0.0, 0.0};
+            if (gm < g.M) {
+                const double *p = pa + (int64_t)gm * sg.A.sr + k;
+                if (VEC && k + 1 < sg.K2) {
+                    a = *(const d2v *)p;
+                } else {
+                    if (k < sg.K2)
+                        a.x = p[0];
+                    if (k + 1 < sg.K2)
+                        a.y = p[1];
+                }
+            }
+            if (gn < g.N) {
+                const double *p = pb + (int64_t)gn * sg.B.sr + k;
+                if (VEC && k + 1 < sg.K2) {
+                    b = *(const d2v *)p;
+                } else {
+                    if (k < sg.K2)
+                        b.x = p[0];
+                    if (k + 1 < sg.K2)
+                        b.y = p[1];
+                }
+            }
+            ra[u] = a;
+            rb[u] = b;
+        }
+    };
+    auto stash = [&](int buf) {
+        char *As = lds[buf], *Bs = As + BM * RB;
+#pragma unroll
+        for (int u = 0; u < NV; u++) {
+            const int q = tid + 256 * u, row = q >> 3, vec = q & 7;
+            *(d2v *)(As + row * RB + ((vec ^ kswz(row)) << 4)) = ra[u];
+            *(d2v *)(Bs + row * RB + ((vec ^ kswz(row)) << 4)) = rb[u];
+        }
+    };
+
+    if (c_begin < c_end)
+        fetch(c_begin);
+    int buf = 0;
+    for (int64_t ch = c_begin; ch < c_end; ch++) {
+        stash(buf);
+        __syncthreads();
+        if (ch + 1 < c_end)
+            fetch(ch + 1);
+        const char *As = lds[buf], *Bs = As + BM * RB;
+        // lane (lr, lq) holds k = 4 lq .. 4 lq + 3 of its row: vectors 2 lq and 2 lq + 1
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            d2v af[BLK], bf[BLK];
+#pragma unroll
+            for (int b = 0; b < BLK; b++) {
+                const int rowa = wm * BLK * 16 + b * 16 + lr, rowb = wn * BLK * 16 + b * 16 + lr;
+                af[b] = *(const d2v *)(As + rowa * RB + (((2 * lq + h) ^ kswz(rowa)) << 4));
+                bf[b] = *(const d2v *)(Bs + rowb * RB + (((2 * lq + h) ^ kswz(rowb)) << 4));
+            }
+#pragma unroll
+            for (int t = 0; t < 2; t++)
+#pragma unroll
+                for (int x = 0; x < BLK; x++)
+#pragma unroll
+                    for (int y = 0; y < BLK; y++)
+                        acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[x][t], bf[y][t], acc[x][y], 0, 0, 0);
+        }
+        buf ^= 1;
+    }
+
+#pragma unroll
+    for (int x = 0; x < BLK; x++)
+#pragma unroll
+        for (int y = 0; y < BLK; y++) {
+            const int n = n0 + wn * BLK * 16 + y * 16 + lr;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int m = m0 + wm * BLK * 16 + x * 16 + lq + 4 * r;
+                if (m < g.M && n < g.N) {
+                    if (g.splits == 1) {
+                        double *cp = g.C + (int64_t)beta * g.sCb + (int64_t)m * g.sCm + (int64_t)n * g.sCn;
+                        *cp += g.alpha * acc[x][y][r];
+                    } else {
+                        g.scratch[((int64_t)split * g.M + m) * g.N + n] = acc[x][y][r];
+                    }
+                }
+            }
+        }
+}
+
+// 0: not K-contiguous; 1: K-contiguous; 2: K-contiguous with 16-byte aligned rows (vector loads)
+static int k_contiguous(const GemmArgs &g) {
+    int r = 2;
+    for (int s = 0; s < g.nseg; s++) {
+        const GemmSeg &sg = g.seg[s];
+        const GemmOp *ops[2] = {&sg.A, &sg.B};
+        for (const GemmOp *op : ops) {
+            if (op->s2 != 1)
+                return 0;
+            if ((op->sr & 1) || (op->s1 & 1) || (op->sb & 1) || (((uintptr_t)op->base) & 15))
+                r = 1;
+        }
+    }
+    return r;
+}
+
 hipError_t launch_gemm(const GemmArgs &g, int batch, hipStream_t s) {
-    const int ntiles = ((g.M + GT - 1) / GT) * ((g.N + GT - 1) / GT);
-    hipLaunchKernelGGL(gemm_kernel, dim3((unsigned)ntiles, (unsigned)batch, (unsigned)g.splits), dim3(256), 0, s, g);
+    if (const int kc = k_contiguous(g)) {
+        // 64x64 tiles: for FP64 MFMA more resident work-groups beat larger register tiles
+        int blk = 2;
+        if (const char *e = std::getenv("SCESS_GEMM_BLK"))  // tuning override: 2 or 4
+            blk = std::atoi(e) == 4 ? 4 : 2;
+        const int bm = 32 * blk;
+        const int nt = ((g.M + bm - 1) / bm) * ((g.N + bm - 1) / bm);
+        const dim3 grid((unsigned)nt, (unsigned)batch, (unsigned)g.splits);
+        if (blk == 4 && kc == 2)
+            hipLaunchKernelGGL((gemm_kc_kernel<4, true>), grid, dim3(256), 0, s, g);
+        else if (blk == 4)
+            hipLaunchKernelGGL((gemm_kc_kernel<4, false>), grid, dim3(256), 0, s, g);
+        else if (kc == 2)
+            hipLaunchKernelGGL((gemm_kc_kernel<2, true>), grid, dim3(256), 0, s, g);
+        else
+            hipLaunchKernelGGL((gemm_kc_kernel<2, false>), grid, dim3(256), 0, s, g);
+    } else {
+        const int ntiles = ((g.M + GT - 1) / GT) * ((g.N + GT - 1) / GT);
+        hipLaunchKernelGGL(gemm_kernel, dim3((unsigned)ntiles, (unsigned)batch, (unsigned)g.splits), dim3(256), 0, s, g);
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || g.splits == 1)
         return e;
