@@ -28,6 +28,7 @@ struct TriplesState {
     int *d_ijk_all = nullptr;
     std::vector<int> orbits;
     int *d_orbits = nullptr;
+    uint64_t *d_orbtab = nullptr;  // orbit_table entries
     int nb = 0, norb = 0, ngrp = 0;  // blocks per dim, orbits, epilogue partials per ijk
     // work buffers
     void *W1 = nullptr, *W2 = nullptr;
@@ -102,6 +103,7 @@ void free_triples(TriplesState &t) {
     dfree(t.ev);
     dfree(t.d_ijk_all);
     dfree(t.d_orbits);
+    dfree(t.d_orbtab);
     dfree(t.W1);
     dfree(t.W2);
     dfree(t.d_work_ijk);
@@ -296,6 +298,12 @@ int scess_t_set_inputs(scess_ctx *c, const scess_t_in *in, int prec_mask) {
             }
     t.norb = (int)(t.orbits.size() / 3);
     t.ngrp = epilogue_groups(t.norb);
+    {
+        std::vector<uint64_t> tab;
+        orbit_table(t.orbits, tab);
+        HIP_TRY(c, hipMalloc((void **)&t.d_orbtab, sizeof(uint64_t) * tab.size()));
+        HIP_TRY(c, hipMemcpy(t.d_orbtab, tab.data(), sizeof(uint64_t) * tab.size(), hipMemcpyHostToDevice));
+    }
     HIP_TRY(c, hipMalloc((void **)&t.d_ijk_all, sizeof(int) * t.ijk_all.size()));
     HIP_TRY(c, hipMemcpy(t.d_ijk_all, t.ijk_all.data(), sizeof(int) * t.ijk_all.size(), hipMemcpyHostToDevice));
     HIP_TRY(c, hipMalloc((void **)&t.d_orbits, sizeof(int) * t.orbits.size()));
@@ -427,6 +435,7 @@ int scess_t_forward_ex(scess_ctx *c, int prec, double *e_t, double *F_ijk, doubl
     ea.vR = (int)vR;
     ea.nb = t.nb;
     ea.orbits = t.d_orbits;
+    ea.orbtab = t.d_orbtab;
     ea.norb = t.norb;
     ea.ngrp = t.ngrp;
     ea.t1 = t.t1;
@@ -658,6 +667,7 @@ int scess_t_backward(scess_ctx *c, int prec, double dE, scess_t_grad *grad) {
     ea.vR = (int)vR;
     ea.nb = t.nb;
     ea.orbits = t.d_orbits;
+    ea.orbtab = t.d_orbtab;
     ea.norb = t.norb;
     ea.ngrp = t.ngrp;
     ea.t1 = t.t1;

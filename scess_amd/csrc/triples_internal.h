@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 namespace scess {
 
@@ -62,6 +63,7 @@ struct EpiArgs {
     int o, v, vR, nb;     // nb = vR / 8 blocks per dimension
     const int *ijk;       // 3 ints per batch entry
     const int *orbits;    // 3 ints per orbit (A >= B >= C block indices)
+    const uint64_t *orbtab;  // 6 packed geometry entries per orbit (orbit_table)
     int norb;
     const double *U;      // [nbatch][3][v][vR] V pieces (launch_vpieces), rows zero-padded
     const double *t1;     // [o][vR], rows zero-padded
@@ -72,6 +74,10 @@ struct EpiArgs {
     int halve;            // entries with i == j read W1 in place of W2 (see WBuildArgs::halve)
 };
 int epilogue_groups(int norb);
+// Packed geometry of each orbit {(A,B,C) and its permutations} for the epilogue: per distinct
+// block d (in the order of the permutations (ABC, BCA, CAB, CBA, ACB, BAC)) its (P,Q,R), the
+// count of distinct blocks and where the five permuted reads of r3 find their Y block.
+void orbit_table(const std::vector<int> &orbits, std::vector<uint64_t> &tab);
 hipError_t launch_epilogue(int prec, const EpiArgs &a, int nbatch, hipStream_t s);
 // V pieces of every batch entry from S[p][q][x][y] = (px|qy) into U[nbatch][3][v][vR]
 hipError_t launch_vpieces(const double *S, const int *ijk, int o, int v, int vR, double *U, int nbatch, hipStream_t s);

@@ -825,6 +825,35 @@ __device__ inline OrbitRow orbit_row(const int *orbits, int orb, int tid) {
     return r;
 }
 
+// The same geometry from the host-built table (one 64-bit entry per orbit and distinct block d):
+// P, Q, R in bits 0-23, the number of distinct blocks in 24-26, and the distinct-block indices of
+// the five permuted reads in 27-41 (3 bits each).
+template <int PB, int PX, int PY>
+__device__ inline OrbitRow orbit_row_tab(const uint64_t *tab, int orb, int tid) {
+    constexpr int EB = 8;
+    OrbitRow r;
+    const int d = tid >> 6, ap = (tid >> 3) & 7, bp = tid & 7;
+    const uint64_t e = tab[(int64_t)orb * 6 + d];
+    const int P = (int)(e & 255u), Q = (int)((e >> 8) & 255u), R = (int)((e >> 16) & 255u);
+    r.nd = (int)((e >> 24) & 7u);
+    r.own = d < r.nd;
+    r.ga = P * EB + ap;
+    r.gb = Q * EB + bp;
+    r.gc0 = R * EB;
+    auto di = [&](int n) { return (int)((e >> (27 + 3 * n)) & 7u); };
+    r.off[0] = di(0) * PB + bp * PX + ap;
+    r.step[0] = PY;
+    r.off[1] = di(1) * PB + ap * PY + bp;
+    r.step[1] = PX;
+    r.off[2] = di(2) * PB + bp * PY + ap;
+    r.step[2] = PX;
+    r.off[3] = di(3) * PB + ap * PX + bp;
+    r.step[3] = PY;
+    r.off[4] = di(4) * PB + bp * PX + ap * PY;
+    r.step[4] = 1;
+    return r;
+}
+
 // One work-group per (ijk, run of EPI_RUN consecutive orbits).
 constexpr int EPI_RUN = 8;
 
@@ -858,7 +887,7 @@ __global__ __launch_bounds__(EPI_THREADS) void epilogue_kernel(EpiArgs a) {
     double Wn[EB], Yn[EB], En[EB];
     OrbitRow rn;
     auto fetch = [&](int orb) {
-        rn = orbit_row<PB, PX, PY>(a.orbits, orb, tid);
+        rn = orbit_row_tab<PB, PX, PY>(a.orbtab, orb, tid);
         if (rn.own && rn.ga < v && rn.gb < v) {
             // W1 block (P,Q,R) row (a', b') and W2 block (Q,P,R) row (b', a'), 8 contiguous c'
             const int P = rn.ga >> 3, Q = rn.gb >> 3, R = rn.gc0 >> 3;
@@ -909,15 +938,18 @@ __global__ __launch_bounds__(EPI_THREADS) void epilogue_kernel(EpiArgs a) {
                 Yl[d * PB + ap * PX + bp * PY + cp] = Yn[cp];
         __syncthreads();
         if (r.own && r.ga < v && r.gb < v) {
+            // the row's (at most 8) terms in plain FP64, the rows compensated (Neumaier)
+            double rs = 0.0;
 #pragma unroll
             for (int cp = 0; cp < EB; cp++) {
                 if (r.gc0 + cp < v) {  // Yl of masked columns is finite: padding is zero
                     const double r3 = 4.0 * Yn[cp] + Yl[r.off[0] + cp * r.step[0]] + Yl[r.off[1] + cp * r.step[1]] -
                                       2.0 * (Yl[r.off[2] + cp * r.step[2]] + Yl[r.off[3] + cp * r.step[3]] +
                                              Yl[r.off[4] + cp * r.step[4]]);
-                    nadd(s, c, Wn[cp] * r3 * recip(En[cp]));
+                    rs += Wn[cp] * r3 * recip(En[cp]);
                 }
             }
+            nadd(s, c, rs);
         }
         __syncthreads();
     }
@@ -942,6 +974,49 @@ __global__ __launch_bounds__(EPI_THREADS) void epilogue_kernel(EpiArgs a) {
 }
 
 int epilogue_groups(int norb) { return (norb + EPI_RUN - 1) / EPI_RUN; }
+
+void orbit_table(const std::vector<int> &orbits, std::vector<uint64_t> &tab) {
+    const size_t norb = orbits.size() / 3;
+    tab.assign(norb * 6, 0);
+    for (size_t orb = 0; orb < norb; orb++) {
+        const int A = orbits[3 * orb], B = orbits[3 * orb + 1], C = orbits[3 * orb + 2];
+        const int bl1 = B != A ? B : C;
+        auto slot = [&](int blk) { return blk == A ? 0 : (blk == bl1 ? 1 : 2); };
+        const int sB = slot(B), sC = slot(C);
+        // distinct blocks in permutation order, keyed by their slot pattern
+        int key_of[27];
+        for (int &x : key_of)
+            x = -1;
+        int blk[6][3], sl[6][3], nd = 0;
+        const int perms[6][3] = {{A, B, C}, {B, C, A}, {C, A, B}, {C, B, A}, {A, C, B}, {B, A, C}};
+        const int pslots[6][3] = {{0, sB, sC}, {sB, sC, 0}, {sC, 0, sB}, {sC, sB, 0}, {0, sC, sB}, {sB, 0, sC}};
+        for (int p = 0; p < 6; p++) {
+            const int key = pslots[p][0] * 9 + pslots[p][1] * 3 + pslots[p][2];
+            if (key_of[key] < 0) {
+                key_of[key] = nd;
+                for (int x = 0; x < 3; x++) {
+                    blk[nd][x] = perms[p][x];
+                    sl[nd][x] = pslots[p][x];
+                }
+                nd++;
+            }
+        }
+        auto dindex = [&](int s0, int s1, int s2) { return key_of[s0 * 9 + s1 * 3 + s2]; };
+        for (int d = 0; d < 6; d++) {
+            uint64_t e = (uint64_t)nd << 24;
+            if (d < nd) {
+                const int sP = sl[d][0], sQ = sl[d][1], sR = sl[d][2];
+                e |= (uint64_t)blk[d][0] | ((uint64_t)blk[d][1] << 8) | ((uint64_t)blk[d][2] << 16);
+                // the five permuted reads of r3: Y(lb,lc,la), Y(lc,la,lb), Y(lc,lb,la), Y(la,lc,lb), Y(lb,la,lc)
+                const int di[5] = {dindex(sQ, sR, sP), dindex(sR, sP, sQ), dindex(sR, sQ, sP), dindex(sP, sR, sQ),
+                                   dindex(sQ, sP, sR)};
+                for (int n = 0; n < 5; n++)
+                    e |= (uint64_t)di[n] << (27 + 3 * n);
+            }
+            tab[orb * 6 + d] = e;
+        }
+    }
+}
 
 hipError_t launch_epilogue(int prec, const EpiArgs &a, int nbatch, hipStream_t s) {
     const dim3 grid((unsigned)a.ngrp, (unsigned)nbatch);
