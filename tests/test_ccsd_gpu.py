@@ -117,10 +117,15 @@ def test_water_end_to_end_on_gpu(ctx, basis):
     assert abs(cc["E_CCSD"] - cco["E_CCSD"]) < 1e-10
     inp = TriplesInputs(o=o, v=v, e=e, t1=cc["t1"], t2=cc["t2"], ovvv=np.asfortranarray(eri[O, V, V, V]),
                         ovoo=np.asfortranarray(eri[O, V, O, O]), ovov=np.asfortranarray(eri[O, V, O, V]))
-    ctx.set_triples_inputs(inp, ("fp64",))
+    ctx.set_triples_inputs(inp, ("fp64", "fp32", "bf16x3"))
     ET, _ = ctx.triples_forward("fp64")
     assert abs(ET - ref[1]) <= ref[3]
     assert abs(ET - oracle.t_abc(inp)) < 1e-10
+    # at chemical magnitudes (|E(T)| ~ 1e-3 Eh) the mixed-precision contractions stay within 1 microhartree
+    for mode in ("fp32", "bf16x3"):
+        Em, _ = ctx.triples_forward(mode)
+        _say(f"  {basis} {mode}: |dE(T)| = {abs(Em - ET):.3e} Eh")
+        assert abs(Em - ET) <= 1e-6
 
 
 @pytest.mark.parametrize("nao,nmo,phys", [(1, 1, False), (9, 7, False), (9, 9, True), (24, 17, True)])
