@@ -619,6 +619,8 @@ int scess_t_backward(scess_ctx *c, int prec, double dE, scess_t_grad *grad) {
     const int PR = (int)std::min<int64_t>(128, std::max<int64_t>(32, 1024 / tilesR));
     double *dX, *dT, *dRn, *dTM, *dS, *dt1, *GW, *GV, *U, *dU, *dtv, *scr, *out;
     void *W1, *W2;
+    // W of up to WB triples is built by one launch (2 buffers of WB x vR^3 within 4 GB)
+    const int64_t WB = std::max<int64_t>(1, std::min<int64_t>(8, (int64_t)(4.0e9 / (2.0 * (double)wstride * accb))));
     hipError_t he = hipSuccess;
     const size_t nX = (size_t)(o * v * v * vK), nT = (size_t)(o * o * v * vK), nR = (size_t)(o * o * v * oK),
                  nTM = (size_t)(o * v * v * oK), nS = (size_t)(o * o * v * v), nt1 = (size_t)(o * vR),
@@ -632,8 +634,8 @@ int scess_t_backward(scess_ctx *c, int prec, double dE, scess_t_grad *grad) {
         (he = alloc(8 * 3 * (size_t)(v * v), (void **)&dU)) != hipSuccess ||
         (he = alloc(8 * 3 * (size_t)v, (void **)&dtv)) != hipSuccess ||
         (he = alloc(8 * nscr, (void **)&scr)) != hipSuccess || (he = alloc(8 * nout, (void **)&out)) != hipSuccess ||
-        (he = alloc(accb * (size_t)wstride, &W1)) != hipSuccess ||
-        (he = alloc(accb * (size_t)wstride, &W2)) != hipSuccess) {
+        (he = alloc(accb * (size_t)(WB * wstride), &W1)) != hipSuccess ||
+        (he = alloc(accb * (size_t)(WB * wstride), &W2)) != hipSuccess) {
         release();
         return hip_fail(c, he, "scess_t_backward: device allocation");
     }
@@ -657,7 +659,7 @@ int scess_t_backward(scess_ctx *c, int prec, double dE, scess_t_grad *grad) {
     wa.W1 = W1;
     wa.W2 = W2;
     wa.wstride = wstride;
-    wa.halve = 0;  // the adjoint needs the full W1 and W2 of every triple
+    wa.halve = 1;  // triples with i == j build only W1; their W2 is read from W1 (as the forward epilogue)
     EpiArgs ea{};
     ea.W1 = W1;
     ea.W2 = W2;
@@ -697,9 +699,14 @@ int scess_t_backward(scess_ctx *c, int prec, double dE, scess_t_grad *grad) {
         const int occ[3] = {t.work_ijk[3 * w], t.work_ijk[3 * w + 1], t.work_ijk[3 * w + 2]};
         const double deg = (occ[0] == occ[1] || occ[1] == occ[2]) ? 2.0 : 1.0;
         const int *dijk = t.d_work_ijk + 3 * w;
-        wa.ijk = dijk;
+        const int64_t slot = (w - w_begin) % WB;
+        if (slot == 0) {  // W of the next WB triples
+            wa.ijk = dijk;
+            he = launch_wbuild(prec, blk, wbuild_dma(), wa, (int)std::min<int64_t>(WB, w_end - w), c->stream);
+        }
         ea.ijk = dijk;
-        he = launch_wbuild(prec, blk, wbuild_dma(), wa, 1, c->stream);
+        ea.W1 = (const char *)W1 + (size_t)slot * wstride * accb;
+        ea.W2 = occ[0] == occ[1] ? ea.W1 : (const char *)W2 + (size_t)slot * wstride * accb;
         if (he == hipSuccess)
             he = launch_vpieces(t.S, dijk, (int)o, (int)v, (int)vR, U, 1, c->stream);
         if (he == hipSuccess)
