@@ -206,7 +206,8 @@ def ccsd_bench(args):
     ctx = gpu.Context(0)
     t0 = time.perf_counter()
     ctx.set_ccsd_inputs(o, v, ci.e, ci.blocks, chol_vv=ci.chol_vv)
-    log(f"[ccsd] inputs resident in {time.perf_counter() - t0:.1f} s")
+    ctx.set_ccsd_precision(args.ccsd_prec)
+    log(f"[ccsd] inputs resident in {time.perf_counter() - t0:.1f} s; contractions in {args.ccsd_prec}")
     t1 = np.zeros((o, v), order="F")
     t2 = t2_from_ovov(ci.blocks["ovov"], ci.e, o)
     E0 = ctx.ccsd_energy(t1, t2)
@@ -223,12 +224,14 @@ def ccsd_bench(args):
     sec = sum(times) / len(times)
     ladder = 2.0 * o * o * float(v) ** 4
     line = {
-        "metric": "CCSD amplitude update seconds (FP64, Cholesky-regenerated ladder)",
+        "metric": "CCSD amplitude update seconds (Cholesky-regenerated ladder)",
         "value": sec, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": sec * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
+        "dtype": {"fp64": "f64", "fp32": "f32 products, FP64 accumulation and update",
+                  "bf16x3": "bf16x3 products, FP64 accumulation and update"}[args.ccsd_prec],
+        "data": "synthetic",
         "config": {"workload": f"closed-shell CCSD update from MP2 amplitudes, o={o}, v={v}, naux={ci.naux}, "
-                               f"seed {args.seed}", "o": o, "v": v},
+                               f"seed {args.seed}", "o": o, "v": v, "contraction_precision": args.ccsd_prec},
         "E_MP2": E0, "E_after_one_update": E1,
         "ladder_flops": ladder, "ladder_regeneration_flops": 2.0 * float(v) ** 4 * ci.naux,
         "tflops_reference_formulation": (ladder + 16.0 * o ** 3 * float(v) ** 3 + 4.0 * o * float(v) ** 4) / sec / 1e12,
@@ -256,6 +259,8 @@ def main():
                     help="time only shard S of N equal ranges of occupied triples (one GPU's share)")
     ap.add_argument("--ccsd", action="store_true",
                     help="time one CCSD amplitude update with the Cholesky ladder (default o=50, v=500)")
+    ap.add_argument("--ccsd-prec", default="fp64", choices=["fp64", "fp32", "bf16x3"],
+                    help="product precision of the CCSD residual contractions (with --ccsd)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

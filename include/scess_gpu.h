@@ -173,6 +173,12 @@ int scess_ccsd_energy(scess_ctx *ctx, const double *t1, const double *t2, double
 int scess_ccsd_solve(scess_ctx *ctx, double *t1_out, double *t2_out, scess_ccsd_result *res, scess_ccsd_log log,
                      void *user);
 
+/* Product precision of the residual contractions of scess_ccsd_update / scess_ccsd_solve (no counterpart
+ * in the reference, which is FP64 throughout): SCESS_PREC_FP64 (default), SCESS_PREC_FP32 or
+ * SCESS_PREC_BF16X3. Inputs are rounded in registers, each 16-deep K chunk is summed on the MFMA in FP32
+ * and accumulated in FP64; the amplitude update, energies and DIIS stay FP64. */
+int scess_ccsd_set_precision(scess_ctx *ctx, int prec);
+
 /* TransformTEI (IntegralTransforms.cpp:29-59): (pq|rs) = sum C(mu,p) C(nu,q) C(kap,r) C(lam,s) (mu nu|kap lam),
  * one index at a time on the GPU (four FP64 GEMMs). tei is the nao^4 AO tensor in chemists' notation, C is
  * nao x nmo, out is nmo^4; with chem_to_phys != 0, out(p,r,q,s) = (pq|rs) = <pr|qs>. ColMajor throughout. */

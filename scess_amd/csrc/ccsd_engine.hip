@@ -310,8 +310,22 @@ void Engine::einsum(const std::string &spec, const DTensor &A, const DTensor *B,
         g.sCm = N;
         g.sCn = 1;
         g.sCb = M * N;
+        g.prec = prec;
         if (M > 0x7fffffff || N > 0x7fffffff || K > 0x7fffffff || nb > 65535)
             throw std::runtime_error("einsum " + spec + ": extent too large");
+        // few output tiles over a long K (e.g. Foo, the T1 terms): split K so that ~512 work-groups
+        // run; the partial tiles are added in split order (deterministic)
+        const int64_t tiles = ((M + 63) / 64) * ((N + 63) / 64);
+        DTensor scr;
+        if (nb == 1 && tiles < 256 && K >= 8192) {
+            g.splits = (int)std::min<int64_t>({512 / tiles, K / 2048, 256});
+            if (g.splits > 1) {
+                scr = alloc({g.splits, M, N});
+                g.scratch = scr.p;
+            } else {
+                g.splits = 1;
+            }
+        }
         check(launch_gemm(g, (int)nb, stream), "einsum gemm");
     }
     permute(Lb + Lm + Ln, Cp.p, dims_of(Lb + Lm + Ln), lc, C.p, C.dims, alpha, beta);
@@ -382,6 +396,7 @@ void CcsdState::ladder_from_factor(Engine &E, const DTensor &tau, DTensor &R) {
             g.seg[0].B = {Lvv.p + b0 * v * nP, nP, 0, 1, 0};
             check(hipMemsetAsync(I.p, 0, sizeof(double) * (size_t)I.size(), E.stream), "memset I");
             g.C = I.p;
+            g.prec = E.prec;
             g.sCm = tb * v;
             g.sCn = 1;
             g.sCb = 0;
@@ -397,6 +412,7 @@ void CcsdState::ladder_from_factor(Engine &E, const DTensor &tau, DTensor &R) {
             h.seg[0].K2 = (int)v;
             h.seg[0].A = {I.p, v, tb * v, 1, v * tb * v};
             h.seg[0].B = {tau.p, v * v, v, 1, 0};
+            h.prec = E.prec;
             h.C = R.p + a0 * v + b0;
             h.sCb = v;
             h.sCm = 1;

@@ -28,6 +28,7 @@ struct DTensor {
 struct Engine {
     hipStream_t stream = nullptr;
     double *dotbuf = nullptr;
+    int prec = 0;  // product precision of the einsum contractions (P64, P32, PB3), see GemmArgs::prec
     ~Engine();
     DTensor alloc(std::vector<int64_t> dims, bool zero = false);
     void upload(DTensor &t, const double *host);

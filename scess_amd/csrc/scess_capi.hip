@@ -49,6 +49,7 @@ struct scess_ctx {
     void *user = nullptr;
     int64_t nbatch_req = 0;
     int64_t shard = 0, nshards = 1;  // (T) evaluates work items [n*shard/nshards, n*(shard+1)/nshards)
+    int ccsd_prec = 0;               // product precision of the CCSD residual contractions
     TriplesState t;
     scess::Engine eng;
     scess::CcsdState cc;
@@ -862,6 +863,13 @@ int engine_fail(scess_ctx *c, const std::exception &ex) {
     return fail(c, code, m);
 }
 
+// Contraction precision of the engine for one scope (FP64 again afterwards, also on exceptions)
+struct PrecScope {
+    scess::Engine &E;
+    PrecScope(scess::Engine &e, int prec) : E(e) { E.prec = prec; }
+    ~PrecScope() { E.prec = 0; }
+};
+
 // ColMajor host tensor (labels in index order) -> row-major device tensor with the same label order
 scess::DTensor upload_colmajor(scess::Engine &E, const double *h, std::vector<int64_t> dims) {
     std::vector<int64_t> rev(dims.rbegin(), dims.rend());
@@ -1006,6 +1014,7 @@ int scess_ccsd_update(scess_ctx *c, const double *t1, const double *t2, double *
         scess::CcsdState &S = c->cc;
         scess::DTensor d1 = upload_colmajor(E, t1, {S.o, S.v});
         scess::DTensor d2 = upload_colmajor(E, t2, {S.o, S.o, S.v, S.v});
+        const PrecScope ps(E, c->ccsd_prec);
         S.update(E, d1, d2);
         download_colmajor(E, d1, t1_new);
         download_colmajor(E, d2, t2_new);
@@ -1032,6 +1041,15 @@ int scess_ccsd_energy(scess_ctx *c, const double *t1, const double *t2, double *
     } catch (const std::exception &ex) {
         return engine_fail(c, ex);
     }
+    return SCESS_OK;
+}
+
+int scess_ccsd_set_precision(scess_ctx *c, int prec) {
+    if (!c)
+        return SCESS_ERR_ARG;
+    if (prec < SCESS_PREC_FP64 || prec > SCESS_PREC_BF16X3)
+        return fail(c, SCESS_ERR_ARG, "scess_ccsd_set_precision: unknown precision");
+    c->ccsd_prec = prec;
     return SCESS_OK;
 }
 
@@ -1109,6 +1127,7 @@ int scess_ccsd_solve(scess_ctx *c, double *t1_out, double *t2_out, scess_ccsd_re
             }
             HIP_TRY(c, hipMemcpyAsync(prev.p, t1.p, 8 * n1, hipMemcpyDeviceToDevice, c->stream));
             HIP_TRY(c, hipMemcpyAsync(prev.p + n1, t2.p, 8 * n2, hipMemcpyDeviceToDevice, c->stream));
+            const PrecScope ps(E, c->ccsd_prec);
             S.update(E, t1, t2);
             HIP_TRY(c, hipMemcpyAsync(upd.p, t1.p, 8 * n1, hipMemcpyDeviceToDevice, c->stream));
             HIP_TRY(c, hipMemcpyAsync(upd.p + n1, t2.p, 8 * n2, hipMemcpyDeviceToDevice, c->stream));

@@ -333,7 +333,17 @@ typedef double d2v __attribute__((ext_vector_type(2)));
 
 __device__ inline int kswz(int r) { return (r ^ (r >> 3)) & 7; }
 
-template <int BLK, bool VEC>
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef short s4v __attribute__((ext_vector_type(4)));
+
+__device__ inline short bf16_bits(float f) {  // round to nearest even
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (short)(u >> 16);
+}
+__device__ inline float bf16_value(short h) { return __uint_as_float(((uint32_t)(uint16_t)h) << 16); }
+
+template <int BLK, bool VEC, int PREC>
 __global__ __launch_bounds__(256) void gemm_kc_kernel(GemmArgs g) {
     constexpr int BM = 2 * BLK * 16;   // tile rows (and columns)
     constexpr int RB = 128;            // bytes of one 16-double K chunk of a row
@@ -424,22 +434,78 @@ __global__ __launch_bounds__(256) void gemm_kc_kernel(GemmArgs g) {
             fetch(ch + 1);
         const char *As = lds[buf], *Bs = As + BM * RB;
         // lane (lr, lq) holds k = 4 lq .. 4 lq + 3 of its row: vectors 2 lq and 2 lq + 1
+        if constexpr (PREC == P64) {
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
-            d2v af[BLK], bf[BLK];
+            for (int h = 0; h < 2; h++) {
+                d2v af[BLK], bf[BLK];
+#pragma unroll
+                for (int b = 0; b < BLK; b++) {
+                    const int rowa = wm * BLK * 16 + b * 16 + lr, rowb = wn * BLK * 16 + b * 16 + lr;
+                    af[b] = *(const d2v *)(As + rowa * RB + (((2 * lq + h) ^ kswz(rowa)) << 4));
+                    bf[b] = *(const d2v *)(Bs + rowb * RB + (((2 * lq + h) ^ kswz(rowb)) << 4));
+                }
+#pragma unroll
+                for (int t = 0; t < 2; t++)
+#pragma unroll
+                    for (int x = 0; x < BLK; x++)
+#pragma unroll
+                        for (int y = 0; y < BLK; y++)
+                            acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[x][t], bf[y][t], acc[x][y], 0, 0, 0);
+            }
+        } else {
+            // the lane's 4 k values of each block row, rounded (FP32) or split (hi + lo BF16)
+            f4v a32[BLK], b32[BLK];
 #pragma unroll
             for (int b = 0; b < BLK; b++) {
                 const int rowa = wm * BLK * 16 + b * 16 + lr, rowb = wn * BLK * 16 + b * 16 + lr;
-                af[b] = *(const d2v *)(As + rowa * RB + (((2 * lq + h) ^ kswz(rowa)) << 4));
-                bf[b] = *(const d2v *)(Bs + rowb * RB + (((2 * lq + h) ^ kswz(rowb)) << 4));
+                const d2v a0 = *(const d2v *)(As + rowa * RB + (((2 * lq) ^ kswz(rowa)) << 4));
+                const d2v a1 = *(const d2v *)(As + rowa * RB + (((2 * lq + 1) ^ kswz(rowa)) << 4));
+                const d2v b0 = *(const d2v *)(Bs + rowb * RB + (((2 * lq) ^ kswz(rowb)) << 4));
+                const d2v b1 = *(const d2v *)(Bs + rowb * RB + (((2 * lq + 1) ^ kswz(rowb)) << 4));
+                a32[b] = f4v{(float)a0.x, (float)a0.y, (float)a1.x, (float)a1.y};
+                b32[b] = f4v{(float)b0.x, (float)b0.y, (float)b1.x, (float)b1.y};
             }
+            f4v c32[BLK][BLK];
 #pragma unroll
-            for (int t = 0; t < 2; t++)
+            for (int x = 0; x < BLK; x++)
+#pragma unroll
+                for (int y = 0; y < BLK; y++)
+                    c32[x][y] = f4v{0.f, 0.f, 0.f, 0.f};
+            if constexpr (PREC == P32) {
+#pragma unroll
+                for (int t = 0; t < 4; t++)
+#pragma unroll
+                    for (int x = 0; x < BLK; x++)
+#pragma unroll
+                        for (int y = 0; y < BLK; y++)
+                            c32[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(a32[x][t], b32[y][t], c32[x][y], 0, 0, 0);
+            } else {
+                s4v ah[BLK], al[BLK], bh[BLK], bl[BLK];
+#pragma unroll
+                for (int b = 0; b < BLK; b++)
+#pragma unroll
+                    for (int t = 0; t < 4; t++) {
+                        ah[b][t] = bf16_bits(a32[b][t]);
+                        al[b][t] = bf16_bits(a32[b][t] - bf16_value(ah[b][t]));
+                        bh[b][t] = bf16_bits(b32[b][t]);
+                        bl[b][t] = bf16_bits(b32[b][t] - bf16_value(bh[b][t]));
+                    }
 #pragma unroll
                 for (int x = 0; x < BLK; x++)
 #pragma unroll
-                    for (int y = 0; y < BLK; y++)
-                        acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[x][t], bf[y][t], acc[x][y], 0, 0, 0);
+                    for (int y = 0; y < BLK; y++) {
+                        c32[x][y] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al[x], bh[y], c32[x][y], 0, 0, 0);
+                        c32[x][y] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[x], bl[y], c32[x][y], 0, 0, 0);
+                        c32[x][y] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[x], bh[y], c32[x][y], 0, 0, 0);
+                    }
+            }
+#pragma unroll
+            for (int x = 0; x < BLK; x++)
+#pragma unroll
+                for (int y = 0; y < BLK; y++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++)
+                        acc[x][y][r] += (double)c32[x][y][r];
         }
         buf ^= 1;
     }
@@ -451,7 +517,8 @@ __global__ __launch_bounds__(256) void gemm_kc_kernel(GemmArgs g) {
             const int n = n0 + wn * BLK * 16 + y * 16 + lr;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int m = m0 + wm * BLK * 16 + x * 16 + lq + 4 * r;
+                // output layout of the 16x16 MFMAs: row lq + 4 r (FP64), 4 lq + r (FP32 accumulators)
+                const int m = m0 + wm * BLK * 16 + x * 16 + (PREC == P64 ? lq + 4 * r : 4 * lq + r);
                 if (m < g.M && n < g.N) {
                     if (g.splits == 1) {
                         double *cp = g.C + (int64_t)beta * g.sCb + (int64_t)m * g.sCm + (int64_t)n * g.sCn;
@@ -484,19 +551,19 @@ hipError_t launch_gemm(const GemmArgs &g, int batch, hipStream_t s) {
     if (const int kc = k_contiguous(g)) {
         // 64x64 tiles: for FP64 MFMA more resident work-groups beat larger register tiles
         int blk = 2;
-        if (const char *e = std::getenv("SCESS_GEMM_BLK"))  // tuning override: 2 or 4
-            blk = std::atoi(e) == 4 ? 4 : 2;
+        if (const char *e = std::getenv("SCESS_GEMM_BLK"))  // tuning override: 2 or 4 (FP64 only)
+            blk = std::atoi(e) == 4 && g.prec == P64 ? 4 : 2;
         const int bm = 32 * blk;
         const int nt = ((g.M + bm - 1) / bm) * ((g.N + bm - 1) / bm);
         const dim3 grid((unsigned)nt, (unsigned)batch, (unsigned)g.splits);
-        if (blk == 4 && kc == 2)
-            hipLaunchKernelGGL((gemm_kc_kernel<4, true>), grid, dim3(256), 0, s, g);
-        else if (blk == 4)
-            hipLaunchKernelGGL((gemm_kc_kernel<4, false>), grid, dim3(256), 0, s, g);
-        else if (kc == 2)
-            hipLaunchKernelGGL((gemm_kc_kernel<2, true>), grid, dim3(256), 0, s, g);
-        else
-            hipLaunchKernelGGL((gemm_kc_kernel<2, false>), grid, dim3(256), 0, s, g);
+#define SCESS_KC(B, V, P)                                                                         \
+    if (blk == B && (kc == 2) == V && g.prec == P) {                                              \
+        hipLaunchKernelGGL((gemm_kc_kernel<B, V, P>), grid, dim3(256), 0, s, g);                  \
+    } else
+        SCESS_KC(2, true, P64) SCESS_KC(2, false, P64) SCESS_KC(2, true, P32) SCESS_KC(2, false, P32)
+        SCESS_KC(2, true, PB3) SCESS_KC(2, false, PB3) SCESS_KC(4, true, P64) SCESS_KC(4, false, P64)
+        return hipErrorInvalidValue;
+#undef SCESS_KC
     } else {
         const int ntiles = ((g.M + GT - 1) / GT) * ((g.N + GT - 1) / GT);
         hipLaunchKernelGGL(gemm_kernel, dim3((unsigned)ntiles, (unsigned)batch, (unsigned)g.splits), dim3(256), 0, s, g);

@@ -109,6 +109,10 @@ struct GemmArgs {
     double alpha;
     int splits;
     double *scratch;  // splits * M * N
+    // products in P64 (default), P32 or PB3 on the K-contiguous path; FP64 inputs are rounded (split
+    // into hi + lo bf16 for PB3) in registers, each 16-deep K chunk is summed on the MFMA in FP32
+    // and added to FP64 accumulators. Other layouts always use FP64.
+    int prec;
 };
 hipError_t launch_gemm(const GemmArgs &g, int batch, hipStream_t s);
 // G_W, G_V ([vR]^3 row-major, FP64) of the single occupied triple a.ijk; W in a.W1/W2, V pieces in a.U

@@ -119,6 +119,8 @@ def load(path: str = LIB_PATH):
     lib.scess_ccsd_energy.restype = c.c_int
     lib.scess_ccsd_solve.argtypes = [c.c_void_p, _dp, _dp, c.POINTER(CcsdResult), LOG_FN, c.c_void_p]
     lib.scess_ccsd_solve.restype = c.c_int
+    lib.scess_ccsd_set_precision.argtypes = [c.c_void_p, c.c_int]
+    lib.scess_ccsd_set_precision.restype = c.c_int
     lib.scess_transform_tei.argtypes = [c.c_void_p, c.c_int64, c.c_int64, _dp, _dp, c.c_int, _dp]
     lib.scess_transform_tei.restype = c.c_int
     _LIB = lib
@@ -250,6 +252,10 @@ class Context:
                      _ptr(drv) if drv is not None else None, _ptr(fen) if fen is not None else None)
         self.check(self.lib.scess_ccsd_set_inputs(self.h, ctypes.byref(cin)))
         self.cc_o, self.cc_v = int(o), int(v)
+
+    def set_ccsd_precision(self, precision="fp64"):
+        """Product precision of the CCSD residual contractions ("fp64", "fp32", "bf16x3")."""
+        self.check(self.lib.scess_ccsd_set_precision(self.h, PRECISIONS[precision]))
 
     def ccsd_update(self, t1, t2):
         t1 = _f64(t1)

@@ -215,3 +215,40 @@ def test_water_baseline_geometry_gpu_vs_oracle(ctx):
     ET, _ = ctx.triples_forward("fp64")
     assert abs(ET - oracle.t_abc(inp)) < 1e-10
     _say(f"  E(CCSD)={cc['E_CCSD']!r} E(T)={ET!r}")
+
+
+@pytest.mark.parametrize("mode,tol", [("fp32", 1e-5), ("bf16x3", 1e-4)])
+def test_mixed_precision_contractions(ctx, mode, tol):
+    """scess_ccsd_set_precision: one update with FP32 / split-BF16x3 products against the FP64 one
+    (relative to the largest amplitude), and the water cc-pVDZ CCSD energy within 1 microhartree."""
+    _say(f"case ccsd mixed {mode}")
+    o, v = 5, 24
+    eri, e, blocks, chol = _system(o, v, 9, 0.3)
+    ctx.set_ccsd_inputs(o, v, e, blocks, chol_vv=chol)
+    rng = np.random.default_rng(3)
+    t1 = np.asfortranarray(rng.standard_normal((o, v)) * 1e-2)
+    t2 = rng.standard_normal((o, o, v, v)) * 1e-2
+    t2 = np.asfortranarray(t2 + t2.transpose(1, 0, 3, 2))
+    try:
+        r1, r2 = ctx.ccsd_update(t1, t2)
+        ctx.set_ccsd_precision(mode)
+        m1, m2 = ctx.ccsd_update(t1, t2)
+        scale = np.abs(r2).max()
+        err = max(np.abs(m2 - r2).max(), np.abs(m1 - r1).max()) / scale
+        _say(f"  update rel err {err:.2e}")
+        assert 0.0 < err <= tol
+        from oracle import scf
+        r = scf.molecule_inputs("cc-pVDZ")
+        oo, eri_w, ew = r["nocc"], r["eri"], r["e"]
+        O, V = slice(0, oo), slice(oo, None)
+        bw = {"ovov": eri_w[O, V, O, V], "oovv": eri_w[O, O, V, V], "ovvo": eri_w[O, V, V, O],
+              "ovvv": eri_w[O, V, V, V], "ovoo": eri_w[O, V, O, O], "oooo": eri_w[O, O, O, O],
+              "vvvv": eri_w[V, V, V, V]}
+        ctx.set_ccsd_inputs(oo, ew.size - oo, ew, bw)
+        cm = ctx.ccsd_solve()
+        ctx.set_ccsd_precision("fp64")
+        c64 = ctx.ccsd_solve()
+        _say(f"  water E(CCSD) {mode} - fp64 = {cm['E_CCSD'] - c64['E_CCSD']:.3e} Eh")
+        assert cm["converged"] and abs(cm["E_CCSD"] - c64["E_CCSD"]) < 1e-6
+    finally:
+        ctx.set_ccsd_precision("fp64")
