@@ -127,18 +127,22 @@ def adjoint_bench(args):
     v = args.v if args.v is not None else 200
     inp = make_triples_inputs(o, v, args.seed)
     ctx = gpu.Context(0)
-    ctx.set_triples_inputs(inp, ("fp64",))
-    E, st = ctx.triples_forward("fp64")
-    log(f"[adjoint] o={o} v={v} E(T)={E!r} forward {st['seconds']:.3f} s")
+    shard, nshards = (int(x) for x in args.shard.split("/"))
+    ctx.set_shard(shard, nshards)
+    wprec = args.adjoint_prec
+    ctx.set_triples_inputs(inp, tuple(dict.fromkeys(("fp64", wprec))))
+    E, st = ctx.triples_forward(wprec)
+    log(f"[adjoint] o={o} v={v} shard {shard}/{nshards} E(T)={E!r} forward ({wprec}) {st['seconds']:.3f} s")
     for w in range(args.warmup):
-        ctx.triples_backward("fp64")
+        ctx.triples_backward(wprec)
     times = []
     for s_ in range(args.steps):
         t = time.perf_counter()
-        g = ctx.triples_backward("fp64")
+        g = ctx.triples_backward(wprec)
         times.append(time.perf_counter() - t)
         log(f"[adjoint] backward step {s_}: {times[-1]:.3f} s")
     sec = sum(times) / len(times)
+    fwd_share = st["flops_algorithmic"]
 
     def replace(**kw):
         d = dict(o=inp.o, v=inp.v, e=inp.e, t1=inp.t1, t2=inp.t2, ovvv=inp.ovvv, ovoo=inp.ovoo, ovov=inp.ovov)
@@ -160,8 +164,8 @@ def adjoint_bench(args):
         for sgn in (1.0, -1.0):
             a = np.array(arr, order="F", copy=True)
             a[ix] += sgn * h
-            ctx.set_triples_inputs(replace(**{name: a}), ("fp64",))
-            _, _, P, _ = ctx.triples_forward("fp64", with_F=True, with_partials=True)
+            ctx.set_triples_inputs(replace(**{name: a}), (wprec,))
+            _, _, P, _ = ctx.triples_forward(wprec, with_F=True, with_partials=True)
             parts.append(P)
         diff = (parts[0] - parts[1]) * wgt[:, None]
         fd = float(np.sum(diff[diff != 0.0])) / (2 * h)
@@ -172,16 +176,16 @@ def adjoint_bench(args):
                        "abs_over_max": abs(fd - an) / gmax[name]})
         log(f"[adjoint] fd {n_}: d{name}{tuple(int(x) for x in ix)} analytic {an:.12e} fd {fd:.12e} rel {rel:.2e}")
     ctx.close()
-    fwd_flops = 2.0 * o ** 3 * v ** 3 * (o + v)
     line = {
-        "metric": "(T) adjoint seconds per backward (FP64) and central-difference check",
+        "metric": "(T) adjoint seconds per backward and central-difference check",
         "value": sec, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": sec * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"(T) backward dE/d(vvvo, ooov, ovov, T1, T2), o={o}, v={v}, seed {args.seed}",
-                   "o": o, "v": v},
+        "dtype": "f64 back-contractions, W recomputed in " + wprec, "data": "synthetic",
+        "config": {"workload": f"(T) backward dE/d(vvvo, ooov, ovov, T1, T2), o={o}, v={v}, seed {args.seed}"
+                               + (f", occupied-triple shard {shard} of {nshards}" if nshards > 1 else ""),
+                   "o": o, "v": v, "w_recompute_precision": wprec},
         "E_T": E,
-        "backward_tflops_vs_3x_forward_flops": 3.0 * fwd_flops / sec / 1e12,
+        "backward_tflops_vs_3x_forward_flops": 3.0 * fwd_share / sec / 1e12,
         "fd": {"h": h, "entries": len(checks), "max_rel_err": worst,
                "max_abs_err_over_max_grad": max(c["abs_over_max"] for c in checks) if checks else None,
                "n_rel_err_le_1e-6": sum(c["rel_err"] <= 1e-6 for c in checks), "checks": checks},
@@ -253,6 +257,8 @@ def main():
     ap.add_argument("--adjoint", action="store_true",
                     help="time the (T) backward and check it by central differences (default o=20, v=200)")
     ap.add_argument("--fd-entries", type=int, default=64)
+    ap.add_argument("--adjoint-prec", default="fp64", choices=["fp64", "fp32", "bf16x3"],
+                    help="precision of the W recomputation in the backward (with --adjoint)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (scale for HBM byte counters)")
     ap.add_argument("--shard", default="0/1", metavar="S/N",
