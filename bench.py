@@ -326,7 +326,7 @@ def main():
             "tflops_algorithmic": st["flops_algorithmic"] / sec / 1e12,
             "w_build_ms_per_step": sum(wb) / len(wb),
             "w_build_launches_per_step": wbl[-1],
-            "epilogue_ms_per_step": sum(ep) / len(ep),
+            "epilogue_span_ms_per_step": sum(ep) / len(ep),
             "flops_algorithmic": st["flops_algorithmic"],
             "flops_executed": st["flops_executed"],
             "n_ijk": st["n_ijk"],

@@ -69,7 +69,9 @@ typedef struct scess_t_in {
 typedef struct scess_t_stats {
     double seconds;            /* device time of the whole call (HIP events) */
     double w_build_ms;         /* summed device time of the W-build launches */
-    double epilogue_ms;        /* summed device time of the epilogue launches */
+    double epilogue_ms;        /* summed span of the V-piece, epilogue and partial-sum launches on the
+                                  epilogue stream; they overlap the next batch's W-build, so this is
+                                  not additive with w_build_ms */
     int64_t w_build_launches;
     int64_t epilogue_launches;
     int64_t n_ijk;             /* unique occupied triples i>=j>=k evaluated */

@@ -44,6 +44,7 @@ struct TriplesState {
 struct scess_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;  // (T) epilogue stream, overlapping the next batch's W-build
     std::string err;
     int (*should_stop)(void *) = nullptr;
     void *user = nullptr;
@@ -141,6 +142,11 @@ int scess_create(scess_ctx **out, int device) {
         delete c;
         return SCESS_ERR_HIP;
     }
+    if (hipStreamCreate(&c->stream2) != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return SCESS_ERR_HIP;
+    }
     c->eng.stream = c->stream;
     *out = c;
     return SCESS_OK;
@@ -154,7 +160,9 @@ void scess_destroy(scess_ctx *c) {
     free_triples(c->t);
     c->cc = scess::CcsdState{};
     (void)hipStreamSynchronize(c->stream);
+    (void)hipStreamSynchronize(c->stream2);
     (void)hipStreamDestroy(c->stream);
+    (void)hipStreamDestroy(c->stream2);
     delete c;
 }
 
@@ -376,7 +384,9 @@ int scess_t_forward_ex(scess_ctx *c, int prec, double *e_t, double *F_ijk, doubl
     nbatch = std::max<int64_t>(1, std::min<int64_t>(nbatch, std::max<int64_t>(nshare, 1)));
     if (nbatch > 65535)
         nbatch = 65535;
-    const size_t wneed = (size_t)nbatch * (size_t)wstride * accb;
+    // two sets of W, U and partials: the epilogue of batch b reads set b % 2 while the W-build of
+    // batch b + 1 writes the other one
+    const size_t wneed = 2 * (size_t)nbatch * (size_t)wstride * accb;
     if (t.wbytes < wneed) {
         dfree(t.W1);
         dfree(t.W2);
@@ -386,19 +396,20 @@ int scess_t_forward_ex(scess_ctx *c, int prec, double *e_t, double *F_ijk, doubl
         t.wbytes = wneed;
     }
     // per-orbit partials: whole job if the caller wants them, else one batch
-    const size_t pneed = (size_t)(partials_out ? std::max<int64_t>(nwork, 1) : nbatch) * (size_t)t.ngrp;
+    const size_t pneed = (size_t)(partials_out ? std::max<int64_t>(nwork, 1) : 2 * nbatch) * (size_t)t.ngrp;
     if (t.partials_cap < pneed) {
         dfree(t.partials);
         HIP_TRY(c, hipMalloc((void **)&t.partials, sizeof(double) * pneed));
         t.partials_cap = pneed;
     }
-    const size_t uneed = (size_t)nbatch * 3 * (size_t)(v * vR);
+    const size_t uneed = 2 * (size_t)nbatch * 3 * (size_t)(v * vR);
     if (t.u_cap < uneed) {
         dfree(t.U);
         HIP_TRY(c, hipMalloc((void **)&t.U, sizeof(double) * uneed));
         t.u_cap = uneed;
     }
     HIP_TRY(c, hipMemsetAsync(t.F, 0, sizeof(double) * nall, c->stream));
+    const size_t wset = (size_t)nbatch * (size_t)wstride * accb;  // bytes of one set of W1 (or W2)
 
     const Operands &op = t.ops[prec];
     int blk = wbuild_tile(prec, (int)v);
@@ -462,17 +473,28 @@ int scess_t_forward_ex(scess_ctx *c, int prec, double *e_t, double *F_ijk, doubl
         destroy_events();
         return fail(c, SCESS_ERR_HIP, "hipEventCreate");
     }
+    // a, b: W-build of the batch on stream (its HIP-event duration); e, c: its V pieces, epilogue and
+    // partial reduction on stream2, which waits for b; the W-build two batches later waits for c
     struct BatchEvents {
-        hipEvent_t a, b, c;
+        hipEvent_t a, b, e, c;
     };
     std::vector<BatchEvents> bev;
-    (void)hipEventRecord(e_begin, c->stream);
+    hipStream_t s0 = c->stream, s1 = c->stream2;
+    hipEvent_t e_ready, e_join;
+    if (new_event(&e_ready) != hipSuccess || new_event(&e_join) != hipSuccess) {
+        destroy_events();
+        return fail(c, SCESS_ERR_HIP, "hipEventCreate");
+    }
+    (void)hipEventRecord(e_begin, s0);
+    (void)hipEventRecord(e_ready, s0);  // F cleared
+    (void)hipStreamWaitEvent(s1, e_ready, 0);
 
     int rc = SCESS_OK;
     bool cancelled = false;
     for (int64_t b0 = w_begin; b0 < w_end; b0 += nbatch) {
+        const size_t k = bev.size(), set = k & 1;
         // keep at most two batches in flight so cancellation is polled while the device works
-        if (bev.size() >= 2 && hipEventSynchronize(bev[bev.size() - 2].c) != hipSuccess) {
+        if (k >= 2 && hipEventSynchronize(bev[k - 2].c) != hipSuccess) {
             rc = fail(c, SCESS_ERR_HIP, "hipEventSynchronize");
             break;
         }
@@ -483,26 +505,41 @@ int scess_t_forward_ex(scess_ctx *c, int prec, double *e_t, double *F_ijk, doubl
         const int nb = (int)std::min<int64_t>(nbatch, w_end - b0);
         wa.ijk = t.d_work_ijk + 3 * b0;
         ea.ijk = t.d_work_ijk + 3 * b0;
+        wa.W1 = (char *)t.W1 + set * wset;
+        wa.W2 = (char *)t.W2 + set * wset;
+        ea.W1 = wa.W1;
+        ea.W2 = wa.W2;
+        double *U = t.U + set * (size_t)nbatch * 3 * (size_t)(v * vR);
+        ea.U = U;
         hipError_t he = hipSuccess;
         BatchEvents be{};
-        if (new_event(&be.a) != hipSuccess || new_event(&be.b) != hipSuccess || new_event(&be.c) != hipSuccess) {
+        if (new_event(&be.a) != hipSuccess || new_event(&be.b) != hipSuccess || new_event(&be.e) != hipSuccess ||
+            new_event(&be.c) != hipSuccess) {
             rc = fail(c, SCESS_ERR_HIP, "hipEventCreate");
             break;
         }
-        double *pbase = t.partials + (partials_out ? (size_t)b0 * t.ngrp : 0);
+        double *pbase = t.partials + (partials_out ? (size_t)b0 * t.ngrp : set * (size_t)nbatch * t.ngrp);
         ea.partials = pbase;
-        (void)hipEventRecord(be.a, c->stream);
-        he = launch_wbuild(prec, blk, wbuild_dma(), wa, nb, c->stream);
+        if (k >= 2)
+            he = hipStreamWaitEvent(s0, bev[k - 2].c, 0);  // this W set is no longer read
         if (he == hipSuccess)
-            he = hipEventRecord(be.b, c->stream);
+            he = hipEventRecord(be.a, s0);
         if (he == hipSuccess)
-            he = launch_vpieces(t.S, ea.ijk, (int)o, (int)v, (int)vR, t.U, nb, c->stream);
+            he = launch_wbuild(prec, blk, wbuild_dma(), wa, nb, s0);
         if (he == hipSuccess)
-            he = launch_epilogue(prec, ea, nb, c->stream);
+            he = hipEventRecord(be.b, s0);
         if (he == hipSuccess)
-            he = hipEventRecord(be.c, c->stream);
+            he = hipStreamWaitEvent(s1, be.b, 0);
         if (he == hipSuccess)
-            he = launch_reduce_partials(pbase, t.ngrp, t.d_work_index + b0, t.F, nb, c->stream);
+            he = hipEventRecord(be.e, s1);
+        if (he == hipSuccess)
+            he = launch_vpieces(t.S, ea.ijk, (int)o, (int)v, (int)vR, U, nb, s1);
+        if (he == hipSuccess)
+            he = launch_epilogue(prec, ea, nb, s1);
+        if (he == hipSuccess)
+            he = launch_reduce_partials(pbase, t.ngrp, t.d_work_index + b0, t.F, nb, s1);
+        if (he == hipSuccess)
+            he = hipEventRecord(be.c, s1);
         if (he != hipSuccess) {
             rc = hip_fail(c, he, "kernel launch");
             break;
@@ -510,12 +547,16 @@ int scess_t_forward_ex(scess_ctx *c, int prec, double *e_t, double *F_ijk, doubl
         bev.push_back(be);
     }
     if (rc == SCESS_OK && !cancelled) {
-        hipError_t he = launch_reduce_energy(t.F, t.d_ijk_all, (int)nall, t.E, c->stream);
+        hipError_t he = launch_reduce_energy(t.F, t.d_ijk_all, (int)nall, t.E, s1);
         if (he != hipSuccess)
             rc = hip_fail(c, he, "reduce_energy launch");
     }
-    (void)hipEventRecord(e_end, c->stream);
-    hipError_t se = hipStreamSynchronize(c->stream);
+    (void)hipEventRecord(e_join, s1);
+    (void)hipStreamWaitEvent(s0, e_join, 0);
+    (void)hipEventRecord(e_end, s0);
+    hipError_t se = hipStreamSynchronize(s0);
+    if (se == hipSuccess)
+        se = hipStreamSynchronize(s1);
     if (rc == SCESS_OK && se != hipSuccess)
         rc = hip_fail(c, se, "scess_t_forward");
     if (rc == SCESS_OK && cancelled)
@@ -548,7 +589,7 @@ int scess_t_forward_ex(scess_ctx *c, int prec, double *e_t, double *F_ijk, doubl
             float m1 = 0.0f, m2 = 0.0f;
             if (hipEventElapsedTime(&m1, be.a, be.b) == hipSuccess)
                 stats->w_build_ms += m1;
-            if (hipEventElapsedTime(&m2, be.b, be.c) == hipSuccess)
+            if (hipEventElapsedTime(&m2, be.e, be.c) == hipSuccess)
                 stats->epilogue_ms += m2;
         }
         stats->w_build_launches = (int64_t)bev.size();
