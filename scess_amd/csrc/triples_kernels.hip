@@ -523,23 +523,34 @@ __global__ __launch_bounds__(256) void wbuild_dma_kernel(WBuildArgs a) {
 
     int s = 0, kc = 0;
     Seg sg = get_seg(a, fam1, 0, i, j, k, fx);
-    auto issue = [&](int st) {
-        char *base = lds[st];
+    // this lane's load addresses at k-chunk 0 of the current segment (rows past v load row v - 1,
+    // valid memory whose outputs are never stored); a chunk adds kc * RB
+    const char *pa[NIW], *pb[NIW];
+    auto lane_rows = [&]() {
 #pragma unroll
         for (int u = 0; u < NIW; u++) {
-            int gr = r0 + lrow[u], gc = c0 + lrow[u];
-            gr = gr < v ? gr : v - 1;  // rows past v load a valid row; their outputs are never stored
-            gc = gc < v ? gc : v - 1;
-            const int64_t off = (int64_t)kc * RB + lvoff[u];
-            __builtin_amdgcn_global_load_lds((const void *)(sg.A + gr * sg.lda + off),
+            const int gr = min(r0 + lrow[u], v - 1), gc = min(c0 + lrow[u], v - 1);
+            pa[u] = sg.A + gr * sg.lda + lvoff[u];
+            pb[u] = sg.B + gc * sg.ldb + lvoff[u];
+        }
+    };
+    lane_rows();
+    auto issue = [&](int st) {
+        char *base = lds[st];
+        const int64_t off = (int64_t)kc * RB;
+#pragma unroll
+        for (int u = 0; u < NIW; u++) {
+            __builtin_amdgcn_global_load_lds((const void *)(pa[u] + off),
                                              (lds_void *)(base + (wave * NIW + u) * 1024), 16, 0, 0);
-            __builtin_amdgcn_global_load_lds((const void *)(sg.B + gc * sg.ldb + off),
+            __builtin_amdgcn_global_load_lds((const void *)(pb[u] + off),
                                              (lds_void *)(base + OPB + (wave * NIW + u) * 1024), 16, 0, 0);
         }
         if (++kc == sg.nch) {
             kc = 0;
-            if (++s < nseg)
+            if (++s < nseg) {
                 sg = get_seg(a, fam1, deg ? halved_seg(s) : s, i, j, k, fx);
+                lane_rows();
+            }
         }
     };
     // fragment address of logical 16-byte vector lv of tile row rr
