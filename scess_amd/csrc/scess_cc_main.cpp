@@ -2,6 +2,7 @@
 //
 //   scess_cc ccsd   <FCIDUMP>
 //   scess_cc ccsd_t <FCIDUMP>
+//   scess_cc hf     <FCIDUMP>   (no GPU: E_HF and the orbital energies from the dump)
 //
 // The reference's CLI (src/SCESS/Application.cpp:23-277, Driver.cpp:203-231) builds the
 // molecule, basis, integrals and RHF itself; this runner starts from canonical RHF molecular
@@ -80,11 +81,12 @@ Dump read_fcidump(const std::string &path) {
 }  // namespace
 
 int main(int argc, char **argv) {
-    if (argc != 3 || (std::string(argv[1]) != "ccsd" && std::string(argv[1]) != "ccsd_t")) {
-        std::fprintf(stderr, "usage: %s ccsd|ccsd_t <FCIDUMP>\n", argv[0]);
+    const std::string method = argc == 3 ? argv[1] : "";
+    if (method != "ccsd" && method != "ccsd_t" && method != "hf") {
+        std::fprintf(stderr, "usage: %s ccsd|ccsd_t|hf <FCIDUMP>\n", argv[0]);
         return 1;
     }
-    const bool triples = std::string(argv[1]) == "ccsd_t";
+    const bool triples = method == "ccsd_t";
     try {
         const Dump d = read_fcidump(argv[2]);
         if (d.nelec % 2 != 0)
@@ -107,6 +109,12 @@ int main(int argc, char **argv) {
             E_HF += d.h[k * n + k] + e[k];
         if (offdiag > 1e-6)
             std::printf("warning: the orbitals are not canonical (max |F_pq|, p != q = %.3e)\n", offdiag);
+        if (method == "hf") {
+            std::cout << std::setprecision(12) << "E(Hartree Fock) = " << E_HF << " hartrees\n";
+            for (int64_t p = 0; p < n; p++)
+                std::cout << "e[" << p << "] = " << e[p] << "\n";
+            return 0;
+        }
         // <pq|rs> = (pr|qs), ColMajor n^4 (the layout of TransformTEI(..., true))
         SCESS::CC::TEITensor eri(n, n, n, n);
         for (int64_t s = 0; s < n; s++)

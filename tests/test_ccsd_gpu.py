@@ -12,6 +12,7 @@ import pytest
 
 from oracle import ccsd_oracle as co
 from scess_amd import gpu
+from fcidump import write_fcidump
 from scess_amd.synthetic import _draw, full_eri_chem
 
 pytestmark = pytest.mark.gpu
@@ -143,23 +144,6 @@ def test_transform_tei(ctx, nao, nmo, phys):
     assert np.abs(out - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max())
 
 
-def _write_fcidump(path, h, eri, nelec, core):
-    """FCIDUMP of MO integrals: h (n x n), chemists' (pq|rs) (C order), each unique element once."""
-    n = h.shape[0]
-    with open(path, "w") as f:
-        f.write(f" &FCI NORB={n},NELEC={nelec},MS2=0,\n  ORBSYM={','.join(['1'] * n)},\n  ISYM=1,\n &END\n")
-        for p in range(n):
-            for q in range(p + 1):
-                for r in range(n):
-                    for s in range(r + 1):
-                        if p * (p + 1) // 2 + q >= r * (r + 1) // 2 + s and eri[p, q, r, s] != 0.0:
-                            f.write(f"{eri[p, q, r, s]:.17e} {p + 1} {q + 1} {r + 1} {s + 1}\n")
-        for p in range(n):
-            for q in range(p + 1):
-                f.write(f"{h[p, q]:.17e} {p + 1} {q + 1} 0 0\n")
-        f.write(f"{core:.17e} 0 0 0 0\n")
-
-
 @pytest.mark.parametrize("basis", ["STO-3G", "cc-pVDZ"])
 def test_cli_ccsd_t_water(tmp_path, basis):
     """scess_cc ccsd_t on a water FCIDUMP: the C++ method API (SolveRCCSD_DIIS, RCCSD_TriplesCorrection)
@@ -177,7 +161,7 @@ def test_cli_ccsd_t_water(tmp_path, basis):
     E_HF, e, C = scf.rhf(S, H, G, 5, scf.nuclear_repulsion(atoms))
     eri = scf.mo_eri(G, C)
     dump = tmp_path / "FCIDUMP"
-    _write_fcidump(dump, C.T @ H @ C, eri, 10, scf.nuclear_repulsion(atoms))
+    write_fcidump(dump, C.T @ H @ C, eri, 10, scf.nuclear_repulsion(atoms))
     r = subprocess.run([cli, "ccsd_t", str(dump)], capture_output=True, text=True, timeout=120)
     _say(r.stdout[-600:], r.stderr[-600:])
     assert r.returncode == 0
