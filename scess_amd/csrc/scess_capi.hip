@@ -753,39 +753,38 @@ int scess_t_backward(scess_ctx *c, int prec, double dE, scess_t_grad *grad) {
             he = launch_vpieces(t.S, dijk, (int)o, (int)v, (int)vR, U, 1, c->stream);
         if (he == hipSuccess)
             he = launch_adjoint_g(prec, ea, dE * 2.0 / deg, GW, GV, c->stream);
+        // dX[p][x][y][f] += sum_z G(x,y,z) T[r][q][z][f]: the two permutation terms with the same
+        // position for x (PERM rows 2m, 2m+1) write the same dX[p], so they run as one GEMM with
+        // two K segments
+        for (int pr = 0; pr < 3 && he == hipSuccess; pr++) {
+            const int64_t p = occ[PERM[2 * pr][0]], sx = pos_stride[PERM[2 * pr][0]];
+            GemmArgs g{};
+            g.nseg = 2;
+            g.M = (int)v;
+            g.N = (int)v;
+            g.alpha = 1.0;
+            g.splits = 1;
+            for (int h = 0; h < 2; h++) {
+                const int sgm = 2 * pr + h;
+                const int64_t q = occ[PERM[sgm][1]], r = occ[PERM[sgm][2]];
+                const int64_t sy = pos_stride[PERM[sgm][1]], sz = pos_stride[PERM[sgm][2]];
+                GemmSeg &sg = g.seg[h];
+                sg.K1 = 1;
+                sg.K2 = (int)v;
+                sg.A = {GW, sx == 1 ? sx : sy, 0, sz, sx == 1 ? sy : sx};  // rows x batch y, or rows y batch x
+                sg.B = {T64 + (r * o + q) * v * vK, 1, 0, vK, 0};
+            }
+            g.C = dX + p * v * v * vK;
+            g.sCn = 1;
+            g.sCb = sx == 1 ? vK : v * vK;
+            g.sCm = sx == 1 ? v * vK : vK;
+            he = launch_gemm(g, (int)v, c->stream);
+        }
         for (int sgm = 0; sgm < 6 && he == hipSuccess; sgm++) {
             const int64_t p = occ[PERM[sgm][0]], q = occ[PERM[sgm][1]], r = occ[PERM[sgm][2]];
             const int64_t sx = pos_stride[PERM[sgm][0]], sy = pos_stride[PERM[sgm][1]], sz = pos_stride[PERM[sgm][2]];
-            // dX[p][x][y][f] += sum_z G(x,y,z) T[r][q][z][f]
-            {
-                GemmArgs g{};
-                g.nseg = 1;
-                g.M = (int)v;
-                g.N = (int)v;
-                g.alpha = 1.0;
-                g.splits = 1;
-                GemmSeg &sg = g.seg[0];
-                sg.K1 = 1;
-                sg.K2 = (int)v;
-                sg.A = {GW, 0, 0, sz, 0};
-                sg.B = {T64 + (r * o + q) * v * vK, 1, 0, vK, 0};
-                g.C = dX + p * v * v * vK;
-                g.sCn = 1;
-                if (sx == 1) {  // batch y, rows x
-                    sg.A.sb = sy;
-                    sg.A.sr = sx;
-                    g.sCb = vK;
-                    g.sCm = v * vK;
-                } else {  // batch x, rows y
-                    sg.A.sb = sx;
-                    sg.A.sr = sy;
-                    g.sCb = v * vK;
-                    g.sCm = vK;
-                }
-                he = launch_gemm(g, (int)v, c->stream);
-            }
             // dT[r][q][z][f] += sum_{x,y} G(x,y,z) X[p][x][y][f]
-            if (he == hipSuccess) {
+            {
                 GemmArgs g{};
                 g.nseg = 1;
                 g.M = (int)v;
@@ -832,31 +831,34 @@ int scess_t_backward(scess_ctx *c, int prec, double dE, scess_t_grad *grad) {
                 g.sCn = 1;
                 he = launch_gemm(g, 1, c->stream);
             }
-            // dTM[r][y][z][m] += sum_x Rn[p][q][x][m] G(x,y,z)
-            if (he == hipSuccess) {
-                GemmArgs g{};
-                g.nseg = 1;
-                g.M = (int)v;
-                g.N = (int)o;
-                g.alpha = 1.0;
-                g.splits = 1;
-                GemmSeg &sg = g.seg[0];
+        }
+        // dTM[r][y][z][m] += sum_x Rn[p][q][x][m] G(x,y,z): the two terms with the same position for z
+        // write the same dTM[r]; one GEMM with two K segments each (rows z and batch y when z is the
+        // contiguous position of G, else rows y and batch z)
+        static const int ZPAIR[3][2] = {{0, 2}, {1, 4}, {3, 5}};
+        for (int pr = 0; pr < 3 && he == hipSuccess; pr++) {
+            const int64_t r = occ[PERM[ZPAIR[pr][0]][2]], sz = pos_stride[PERM[ZPAIR[pr][0]][2]];
+            GemmArgs g{};
+            g.nseg = 2;
+            g.M = (int)v;
+            g.N = (int)o;
+            g.alpha = 1.0;
+            g.splits = 1;
+            for (int h = 0; h < 2; h++) {
+                const int sgm = ZPAIR[pr][h];
+                const int64_t p = occ[PERM[sgm][0]], q = occ[PERM[sgm][1]];
+                const int64_t sx = pos_stride[PERM[sgm][0]], sy = pos_stride[PERM[sgm][1]];
+                GemmSeg &sg = g.seg[h];
                 sg.K1 = 1;
                 sg.K2 = (int)v;
                 sg.B = {Rn64 + (p * o + q) * v * oK, 1, 0, oK, 0};
-                g.C = dTM + r * v * v * oK;
-                g.sCn = 1;
-                if (sy == 1) {  // batch z, rows y
-                    sg.A = {GW, sy, 0, sx, sz};
-                    g.sCb = oK;
-                    g.sCm = v * oK;
-                } else {  // batch y, rows z
-                    sg.A = {GW, sz, 0, sx, sy};
-                    g.sCb = v * oK;
-                    g.sCm = oK;
-                }
-                he = launch_gemm(g, (int)v, c->stream);
+                sg.A = sz == 1 ? GemmOp{GW, sz, 0, sx, sy} : GemmOp{GW, sy, 0, sx, sz};
             }
+            g.C = dTM + r * v * v * oK;
+            g.sCn = 1;
+            g.sCb = sz == 1 ? v * oK : oK;
+            g.sCm = sz == 1 ? oK : v * oK;
+            he = launch_gemm(g, (int)v, c->stream);
         }
         if (he == hipSuccess)
             he = launch_vback(GV, U, t.t1, occ[0], occ[1], occ[2], (int)o, (int)v, (int)vR, dU, dtv, dS, dt1,
