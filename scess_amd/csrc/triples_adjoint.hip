@@ -186,7 +186,8 @@ hipError_t launch_adjoint_g(int prec, const EpiArgs &a, double wgt, double *GW, 
 // 64x64 tiles (2x2 waves of 2x2 blocks), K chunks of 16 along k2 staged through LDS.
 // With splits > 1 (no batch), split p handles a fixed contiguous range of the K chunks
 // and writes its partial tile to scratch[p]; gemm_reduce_kernel adds them in split order.
-constexpr int GT = 64, GKC = 16, GPITCH = GT + 1;
+// pitch 80: the four 16-lane groups of a fragment read land 32 banks apart (conflict-free halves)
+constexpr int GT = 64, GKC = 16, GPITCH = 80;
 
 // One operand's 64 rows x 16 k of chunk (k1, k2c): 4 elements per thread, consecutive along the
 // operand's contiguous dimension so that the loads coalesce.
