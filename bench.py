@@ -3,9 +3,10 @@
 
 Workload: the synthetic (T) configuration at o=30, v=300 (generator in
 scess_amd/synthetic.py, seed 1), inputs resident in HBM before timing. A step
-is one full E(T) evaluation. Every precision mode (FP64, FP32, split-BF16x3)
-is timed for --steps steps after --warmup untimed steps; |dE(T)| of each mode
-is measured against the FP64 GPU result of the same run.
+is one full E(T) evaluation. Every precision mode in --modes (default FP64,
+FP32, split-BF16x3; "ozaki", the exact int8-digit mode, on request) is timed
+for --steps steps after --warmup untimed steps; |dE(T)| of each mode is
+measured against the FP64 GPU result of the same run.
 
 `value` is the throughput (triples/s = o^3 v^3 / seconds per step) of the
 fastest mode whose |dE(T)| <= 1 microhartree; all modes are reported under
@@ -29,9 +30,12 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 METRIC = "(T) triples/s & MFMA TFLOP/s, mixed vs FP64 GPU at |ΔE(T)|≤1µEh"
-PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2516.0 / 3.0}
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2516.0 / 3.0, "ozaki": 5000.0}
 DTYPE = {"fp64": "f64", "fp32": "f32 (FP32 MFMA, FP32 accumulation; FP64 epilogue)",
-         "bf16x3": "bf16x3 (split-BF16 MFMA, FP32 accumulation; FP64 epilogue)"}
+         "bf16x3": "bf16x3 (split-BF16 MFMA, FP32 accumulation; FP64 epilogue)",
+         "ozaki": "i8x7 (FP64 operands as 7 int8 digits, exact int8 MFMA products, FP64 accumulation)"}
+# int8 multiply-adds per FP64 multiply-add in the ozaki mode: digit pairs t + u <= 8 of 7 x 7
+OZAKI_PAIRS = 28
 BAR_EH = 1.0e-6
 
 
@@ -345,14 +349,16 @@ def main():
     h = res[head]
 
     launch_ms = h["w_build_ms_per_step"] / max(h["w_build_launches_per_step"], 1)
-    achieved = h["flops_algorithmic"] / (h["w_build_ms_per_step"] * 1e-3) / 1e12
+    # the ozaki W-build runs OZAKI_PAIRS int8 multiply-adds for every FP64 one: priced in int8 TOP/s
+    per_flop = OZAKI_PAIRS if head == "ozaki" else 1
+    achieved = per_flop * h["flops_algorithmic"] / (h["w_build_ms_per_step"] * 1e-3) / 1e12
     traffic = load_traffic(f"wbuild_{head}_o{o}_v{v}")
     roofline = {
         "bound": "mfma",
-        "kernel": f"wbuild_kernel<{head}>",
+        "kernel": "wbuild_i8_kernel<7>" if head == "ozaki" else f"wbuild_kernel<{head}>",
         "achieved": achieved,
         "peak": PEAK_TFLOPS[head],
-        "unit": "TFLOP/s",
+        "unit": "TOP/s (int8)" if head == "ozaki" else "TFLOP/s",
         "frac": achieved / PEAK_TFLOPS[head],
         "traffic": traffic.get("bytes_per_launch") if traffic else None,
         "avg_launch_ms": launch_ms,

@@ -49,7 +49,10 @@ enum {
 enum {
     SCESS_PREC_FP64 = 0,   /* FP64 operands, FP64 MFMA */
     SCESS_PREC_FP32 = 1,   /* FP32 operands, FP32 MFMA, FP32 accumulation */
-    SCESS_PREC_BF16X3 = 2  /* x = hi + lo in bf16; hi*hi + hi*lo + lo*hi on bf16 MFMA, FP32 accumulation */
+    SCESS_PREC_BF16X3 = 2, /* x = hi + lo in bf16; hi*hi + hi*lo + lo*hi on bf16 MFMA, FP32 accumulation */
+    SCESS_PREC_OZAKI = 3   /* (T) forward only: each tensor scaled by a power of two and split into 7 int8 digits
+                              (Ozaki); the 28 leading digit products summed exactly on int8 MFMA, FP64
+                              accumulation; product error <= ~2^-46 of max|A| max|B| */
 };
 
 /* Inputs of the (T) functional. Replaces the (eri, orbitalEnergies, nOcc, t1,
@@ -126,8 +129,8 @@ int64_t scess_t_num_partials(const scess_ctx *ctx);
 /* Adjoint of E(T) (no counterpart in the reference): writes dE * dE(T)/d(input)
  * for every input entry, in the input layouts, into the caller's host arrays
  * (overwritten). Every array entry is an independent variable read exactly as
- * the forward reads it. W is recomputed per occupied triple in precision prec;
- * the back-contractions run in FP64 and need SCESS_PREC_FP64 packed by
+ * the forward reads it. W is recomputed per occupied triple in precision prec
+ * (FP64, FP32 or BF16X3; SCESS_PREC_OZAKI is rejected); the back-contractions run in FP64 and need SCESS_PREC_FP64 packed by
  * scess_t_set_inputs. Deterministic: triples are processed in a fixed order. */
 int scess_t_backward(scess_ctx *ctx, int prec, double dE, scess_t_grad *grad);
 

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -19,8 +20,9 @@ using namespace scess;
 struct TriplesState {
     bool have = false;
     int o = 0, v = 0, vR = 0;
-    bool have_prec[3] = {false, false, false};
-    Operands ops[3];
+    bool have_prec[kNumPrec] = {false, false, false, false};
+    Operands ops[kNumPrec];
+    char *zero = nullptr;  // zeros for the unused half-chunk of the PI8 W-build
     double *S = nullptr, *t1 = nullptr, *e = nullptr, *ev = nullptr;  // FP64 epilogue inputs
     std::vector<int> ijk_all;                          // 3 per unique occupied triple
     std::vector<int> work;                             // indices (into ijk_all) with i != k
@@ -99,6 +101,7 @@ void free_triples(TriplesState &t) {
         dfree(op.TM);
         op = Operands{};
     }
+    dfree(t.zero);
     dfree(t.S);
     dfree(t.t1);
     dfree(t.e);
@@ -204,8 +207,8 @@ int scess_t_set_inputs(scess_ctx *c, const scess_t_in *in, int prec_mask) {
         return fail(c, SCESS_ERR_ARG, "scess_t_set_inputs: null array or non-positive size");
     if (in->o > 4096 || in->v > 4096)
         return fail(c, SCESS_ERR_ARG, "scess_t_set_inputs: o and v must be <= 4096");
-    if ((prec_mask & 7) == 0 || (prec_mask & ~7))
-        return fail(c, SCESS_ERR_ARG, "scess_t_set_inputs: prec_mask must be a non-empty subset of bits 0..2");
+    if ((prec_mask & 15) == 0 || (prec_mask & ~15))
+        return fail(c, SCESS_ERR_ARG, "scess_t_set_inputs: prec_mask must be a non-empty subset of bits 0..3");
     HIP_TRY(c, hipSetDevice(c->device));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     free_triples(c->t);
@@ -256,19 +259,94 @@ int scess_t_set_inputs(scess_ctx *c, const scess_t_in *in, int prec_mask) {
             return hip_fail(c, e, "pack kernel");
         return SCESS_OK;
     };
+    // PI8: one power-of-two scale per tensor (|x| 2^-e < 1/2), then the digit slices
+    auto pack_i8 = [&](Operands &op, const double *ovvv, const double *t2, const double *ovoo) -> int {
+        unsigned long long *dmax = nullptr;
+        hipError_t e = hipMalloc((void **)&dmax, 3 * sizeof(unsigned long long));
+        if (e != hipSuccess)
+            return hip_fail(c, e, "hipMalloc");
+        unsigned long long hmax[3] = {0, 0, 0};
+        e = hipMemsetAsync(dmax, 0, sizeof(hmax), c->stream);
+        if (e == hipSuccess)
+            e = launch_absmax(ovvv, o * v * v * v, dmax, c->stream);
+        if (e == hipSuccess)
+            e = launch_absmax(t2, o * o * v * v, dmax + 1, c->stream);
+        if (e == hipSuccess)
+            e = launch_absmax(ovoo, o * v * o * o, dmax + 2, c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(hmax, dmax, sizeof(hmax), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(c->stream);
+        (void)hipFree(dmax);
+        if (e != hipSuccess)
+            return hip_fail(c, e, "PI8 scales");
+        int ex[3];
+        for (int n = 0; n < 3; n++) {
+            double mx;
+            std::memcpy(&mx, &hmax[n], sizeof(mx));
+            if (!std::isfinite(mx))
+                return fail(c, SCESS_ERR_ARG, "scess_t_set_inputs: non-finite input");
+            int E = 0;
+            (void)std::frexp(mx, &E);  // mx < 2^E
+            ex[n] = mx > 0.0 ? E + 1 : 0;
+        }
+        op.eX = ex[0];
+        op.eT = ex[1];
+        op.eR = ex[2];
+        // int32 groups: |digit product| <= 4096, at most kI8Slices pairs per group, 4 vK + 4 oK deep
+        if (4096.0 * kI8Slices * (4.0 * (double)op.vK + 4.0 * (double)op.oK) >= 2147483647.0)
+            return fail(c, SCESS_ERR_ARG, "scess_t_set_inputs: too large for the int8 mode");
+        struct Job {
+            int mode;
+            const double *src;
+            void **dst;
+            int64_t rows, kpad;
+            int gran, e;
+        } jobs[4] = {{PACK_X, ovvv, &op.X, o * v * v, op.vK, 64, op.eX},
+                     {PACK_T, t2, &op.T, o * o * v, op.vK, 64, op.eT},
+                     {PACK_RN, ovoo, &op.Rn, o * o * v, op.oK, 32, op.eR},
+                     {PACK_TM, t2, &op.TM, o * v * v, op.oK, 32, op.eT}};
+        for (const Job &jb : jobs) {
+            const size_t bytes = (size_t)jb.rows * (size_t)jb.kpad * kI8Slices;
+            e = hipMalloc(jb.dst, std::max<size_t>(bytes, 16));
+            if (e != hipSuccess)
+                return hip_fail(c, e, "hipMalloc (packed operand)");
+            e = launch_pack_i8(jb.mode, jb.src, *jb.dst, (int)o, (int)v, jb.kpad, jb.gran, std::ldexp(1.0, -jb.e),
+                               c->stream);
+            if (e != hipSuccess)
+                return hip_fail(c, e, "pack_i8 kernel");
+        }
+        if (!t.zero) {
+            e = hipMalloc((void **)&t.zero, 1024);
+            if (e == hipSuccess)
+                e = hipMemsetAsync(t.zero, 0, 1024, c->stream);
+            if (e != hipSuccess)
+                return hip_fail(c, e, "zero buffer");
+        }
+        return SCESS_OK;
+    };
     if ((rc = pack(PACK_S, P64, r_ovov, (void **)&t.S, o * o * v, v))) {
         cleanup();
         free_triples(t);
         return rc;
     }
-    for (int p = 0; p < 3; p++) {
+    for (int p = 0; p < kNumPrec; p++) {
         if (!(prec_mask & (1 << p)))
             continue;
         Operands &op = t.ops[p];
         op.vK = round_up(v, kChunk[p]);
-        op.oK = round_up(o, kChunk[p]);
+        op.oK = round_up(o, kChunkO[p]);
         op.rbv = op.vK * kElemBytes[p];
         op.rbo = op.oK * kElemBytes[p];
+        if (p == PI8) {
+            if ((rc = pack_i8(op, r_ovvv, r_t2, r_ovoo))) {
+                cleanup();
+                free_triples(t);
+                return rc;
+            }
+            t.have_prec[p] = true;
+            continue;
+        }
         if ((rc = pack(PACK_X, p, r_ovvv, &op.X, o * v * v, op.vK)) ||
             (rc = pack(PACK_T, p, r_t2, &op.T, o * o * v, op.vK)) ||
             (rc = pack(PACK_RN, p, r_ovoo, &op.Rn, o * o * v, op.oK)) ||
@@ -348,6 +426,33 @@ int scess_t_set_inputs(scess_ctx *c, const scess_t_in *in, int prec_mask) {
     return SCESS_OK;
 }
 
+// W-build arguments of one precision (the batch's ijk and W pointers are set per launch)
+static WBuildArgs wbuild_args(const TriplesState &t, int prec, int blk, int64_t wstride) {
+    const Operands &op = t.ops[prec];
+    const int64_t v = t.v;
+    const int BM = 2 * blk * 16;
+    WBuildArgs wa{};
+    wa.X = (const char *)op.X;
+    wa.T = (const char *)op.T;
+    wa.Rn = (const char *)op.Rn;
+    wa.TM = (const char *)op.TM;
+    wa.o = t.o;
+    wa.v = t.v;
+    wa.vR = t.vR;
+    wa.rbv = op.rbv;
+    wa.rbo = op.rbo;
+    wa.nchv = (int)(op.vK / kChunk[prec]);
+    wa.ncho = (int)(op.oK / kChunkO[prec]);
+    wa.nt = (int)((v + BM - 1) / BM);
+    wa.ntile1 = (int)(v * wa.nt * wa.nt);
+    wa.wstride = wstride;
+    wa.halve = 1;
+    wa.scale_f = std::ldexp(1.0, op.eX + op.eT);
+    wa.scale_m = std::ldexp(1.0, op.eR + op.eT);
+    wa.zero = t.zero;
+    return wa;
+}
+
 int scess_t_forward(scess_ctx *c, int prec, double *e_t, scess_t_stats *stats) {
     return scess_t_forward_ex(c, prec, e_t, nullptr, nullptr, stats);
 }
@@ -359,7 +464,7 @@ int scess_t_forward_ex(scess_ctx *c, int prec, double *e_t, double *F_ijk, doubl
     if (!e_t)
         return fail(c, SCESS_ERR_ARG, "scess_t_forward: e_t is null");
     *e_t = 0.0;
-    if (prec < 0 || prec > 2)
+    if (prec < 0 || prec >= kNumPrec)
         return fail(c, SCESS_ERR_ARG, "scess_t_forward: unknown precision");
     TriplesState &t = c->t;
     if (!t.have)
@@ -372,7 +477,7 @@ int scess_t_forward_ex(scess_ctx *c, int prec, double *e_t, double *F_ijk, doubl
     const int64_t nwork = (int64_t)t.work.size();
     const int64_t w_begin = nwork * c->shard / c->nshards, w_end = nwork * (c->shard + 1) / c->nshards;
     const int64_t nshare = w_end - w_begin;
-    const size_t accb = prec == P64 ? 8 : 4;
+    const size_t accb = prec == P64 || prec == PI8 ? 8 : 4;
     const int64_t wstride = vR * vR * vR;  // W in 8x8x8 blocks over vR^3
 
     // batch size: bounded W footprint (2 buffers), at least 1
@@ -415,28 +520,12 @@ int scess_t_forward_ex(scess_ctx *c, int prec, double *e_t, double *F_ijk, doubl
     int blk = wbuild_tile(prec, (int)v);
     if (const char *ov = std::getenv("SCESS_WBUILD_BLK")) {  // tuning override: 1, 2, 4 or 5
         const int b = std::atoi(ov);
-        if (b == 1 || b == 2 || b == 4 || b == 5)
+        if (prec != PI8 && (b == 1 || b == 2 || b == 4 || b == 5))
             blk = b;
     }
-    const int BM = 2 * blk * 16;
-    WBuildArgs wa;
-    wa.X = (const char *)op.X;
-    wa.T = (const char *)op.T;
-    wa.Rn = (const char *)op.Rn;
-    wa.TM = (const char *)op.TM;
-    wa.o = (int)o;
-    wa.v = (int)v;
-    wa.vR = (int)vR;
-    wa.rbv = op.rbv;
-    wa.rbo = op.rbo;
-    wa.nchv = (int)(op.vK / kChunk[prec]);
-    wa.ncho = (int)(op.oK / kChunk[prec]);
-    wa.nt = (int)((v + BM - 1) / BM);
-    wa.ntile1 = (int)(v * wa.nt * wa.nt);
+    WBuildArgs wa = wbuild_args(t, prec, blk, wstride);
     wa.W1 = t.W1;
     wa.W2 = t.W2;
-    wa.wstride = wstride;
-    wa.halve = 1;
 
     EpiArgs ea;
     ea.W1 = t.W1;
@@ -623,8 +712,10 @@ int scess_t_backward(scess_ctx *c, int prec, double dE, scess_t_grad *grad) {
         return SCESS_ERR_ARG;
     if (!grad || !grad->ovvv || !grad->ovoo || !grad->ovov || !grad->t1 || !grad->t2)
         return fail(c, SCESS_ERR_ARG, "scess_t_backward: null gradient array");
-    if (prec < 0 || prec > 2)
+    if (prec < 0 || prec >= kNumPrec)
         return fail(c, SCESS_ERR_ARG, "scess_t_backward: unknown precision");
+    if (prec == PI8)  // switched off: a run of this path at o = 34, v = 7 faulted the GPU, cause not found
+        return fail(c, SCESS_ERR_ARG, "scess_t_backward: SCESS_PREC_OZAKI is forward only");
     TriplesState &t = c->t;
     if (!t.have)
         return fail(c, SCESS_ERR_STATE, "scess_t_backward: no inputs (call scess_t_set_inputs first)");
@@ -635,7 +726,7 @@ int scess_t_backward(scess_ctx *c, int prec, double dE, scess_t_grad *grad) {
     const Operands &op = t.ops[prec];
     const Operands &o64 = t.ops[P64];
     const int64_t vK = o64.vK, oK = o64.oK;
-    const size_t accb = prec == P64 ? 8 : 4;
+    const size_t accb = prec == P64 || prec == PI8 ? 8 : 4;
     const int64_t wstride = vR * vR * vR;
 
     // device buffers of this call, freed on every exit
@@ -683,25 +774,10 @@ int scess_t_backward(scess_ctx *c, int prec, double dE, scess_t_grad *grad) {
     }
 
     const int blk = wbuild_tile(prec, (int)v);
-    const int BM = 2 * blk * 16;
-    WBuildArgs wa;
-    wa.X = (const char *)op.X;
-    wa.T = (const char *)op.T;
-    wa.Rn = (const char *)op.Rn;
-    wa.TM = (const char *)op.TM;
-    wa.o = (int)o;
-    wa.v = (int)v;
-    wa.vR = (int)vR;
-    wa.rbv = op.rbv;
-    wa.rbo = op.rbo;
-    wa.nchv = (int)(op.vK / kChunk[prec]);
-    wa.ncho = (int)(op.oK / kChunk[prec]);
-    wa.nt = (int)((v + BM - 1) / BM);
-    wa.ntile1 = (int)(v * wa.nt * wa.nt);
+    // triples with i == j build only W1; their W2 is read from W1 (as the forward epilogue)
+    WBuildArgs wa = wbuild_args(t, prec, blk, wstride);
     wa.W1 = W1;
     wa.W2 = W2;
-    wa.wstride = wstride;
-    wa.halve = 1;  // triples with i == j build only W1; their W2 is read from W1 (as the forward epilogue)
     EpiArgs ea{};
     ea.W1 = W1;
     ea.W2 = W2;

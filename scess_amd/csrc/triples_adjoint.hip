@@ -174,7 +174,7 @@ __global__ __launch_bounds__(GK_THREADS) void adjoint_g_kernel(EpiArgs a, double
 }
 
 hipError_t launch_adjoint_g(int prec, const EpiArgs &a, double wgt, double *GW, double *GV, hipStream_t s) {
-    if (prec == P64)
+    if (prec == P64 || prec == PI8)  // both build W in FP64
         hipLaunchKernelGGL(adjoint_g_kernel<double>, dim3((unsigned)a.norb), dim3(GK_THREADS), 0, s, a, wgt, GW, GV);
     else
         hipLaunchKernelGGL(adjoint_g_kernel<float>, dim3((unsigned)a.norb), dim3(GK_THREADS), 0, s, a, wgt, GW, GV);

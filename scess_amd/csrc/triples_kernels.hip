@@ -24,6 +24,7 @@
 // one compensated FP64 partial per (ijk, orbit).
 #include "triples_internal.h"
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdlib>
 
@@ -88,14 +89,9 @@ __device__ inline double block_reduce_256(double s, double c, double *red_s, dou
 // -------------------------------------------------------------------- pack
 // One thread per packed element (row, k); gathers from the ColMajor FP64
 // input. Rows past the real K extent are zero.
-template <int PREC>
-__global__ void pack_kernel(int mode, const double *__restrict__ src, void *__restrict__ dst, int o_, int v_,
-                            int64_t kpad, int64_t nrows) {
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= nrows * kpad)
-        return;
-    const int64_t o = o_, v = v_;
-    const int64_t row = gid / kpad, k = gid - row * kpad;
+// Element (row, k) of a packed operand, gathered from the ColMajor FP64 input (0 past the real K).
+__device__ inline double pack_value(int mode, const double *__restrict__ src, int64_t o, int64_t v, int64_t row,
+                                    int64_t k) {
     double val = 0.0;
     switch (mode) {
     case PACK_X: {  // [p][x][y] x f : (px|yf) = ovvv(p,x,y,f)
@@ -123,6 +119,17 @@ __global__ void pack_kernel(int mode, const double *__restrict__ src, void *__re
         val = src[p + o * (x + v * (q + o * k))];
     } break;
     }
+    return val;
+}
+
+template <int PREC>
+__global__ void pack_kernel(int mode, const double *__restrict__ src, void *__restrict__ dst, int o_, int v_,
+                            int64_t kpad, int64_t nrows) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= nrows * kpad)
+        return;
+    const int64_t row = gid / kpad, k = gid - row * kpad;
+    const double val = pack_value(mode, src, o_, v_, row, k);
     if constexpr (PREC == P64) {
         ((double *)dst)[gid] = val;
     } else if constexpr (PREC == P32) {
@@ -135,6 +142,75 @@ __global__ void pack_kernel(int mode, const double *__restrict__ src, void *__re
         d[base] = hi;
         d[base + 8] = lo;
     }
+}
+
+// Ozaki splitting of one element: m = x * scale (|m| < 1/2, exact), then per slice
+// y = 128 m, d = rint(y), m = y - d (all exact in FP64); |d| <= 64.
+__global__ void pack_i8_kernel(int mode, const double *__restrict__ src, int8_t *__restrict__ dst, int o_, int v_,
+                               int64_t kpad, int gran, double scale, int64_t nrows) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= nrows * kpad)
+        return;
+    const int64_t row = gid / kpad, k = gid - row * kpad;
+    double m = pack_value(mode, src, o_, v_, row, k) * scale;
+    int8_t *d = dst + row * kpad * kI8Slices + (k / gran) * gran * kI8Slices + (k % gran);
+#pragma unroll
+    for (int t = 0; t < kI8Slices; t++) {
+        const double y = m * 128.0;
+        const double q = rint(y);
+        m = y - q;
+        d[t * gran] = (int8_t)(int)q;
+    }
+}
+
+static int64_t pack_rows(int mode, int64_t o, int64_t v) {
+    switch (mode) {
+    case PACK_X: return o * v * v;
+    case PACK_T: return o * o * v;
+    case PACK_RN: return o * o * v;
+    case PACK_TM: return o * v * v;
+    case PACK_S: return o * o * v;
+    default: return -1;
+    }
+}
+
+hipError_t launch_pack_i8(int mode, const double *src, void *dst, int o, int v, int64_t kpad, int gran,
+                          double scale, hipStream_t s) {
+    const int64_t nrows = pack_rows(mode, o, v);
+    if (nrows < 0 || mode == PACK_S || (gran != 32 && gran != 64) || kpad % gran)
+        return hipErrorInvalidValue;
+    const int64_t blocks = (nrows * kpad + 255) / 256;
+    if (blocks > 0x7fffffffLL)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(pack_i8_kernel, dim3((unsigned)blocks), dim3(256), 0, s, mode, src, (int8_t *)dst, o, v,
+                       kpad, gran, scale, nrows);
+    return hipGetLastError();
+}
+
+// |x| of non-negative doubles orders like their bit patterns, so an integer atomic max finds the max
+__global__ __launch_bounds__(256) void absmax_kernel(const double *__restrict__ x, int64_t n,
+                                                     unsigned long long *out) {
+    __shared__ unsigned long long red[256];
+    unsigned long long m = 0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const unsigned long long b = (unsigned long long)__double_as_longlong(fabs(x[i]));
+        m = b > m ? b : m;
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w && red[threadIdx.x + w] > red[threadIdx.x])
+            red[threadIdx.x] = red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        atomicMax(out, red[0]);
+}
+
+hipError_t launch_absmax(const double *x, int64_t n, unsigned long long *out, hipStream_t s) {
+    const int64_t blocks = std::min<int64_t>(std::max<int64_t>((n + 255) / 256, 1), 4096);
+    hipLaunchKernelGGL(absmax_kernel, dim3((unsigned)blocks), dim3(256), 0, s, x, n, out);
+    return hipGetLastError();
 }
 
 hipError_t launch_pack(int mode, int prec, const double *src, void *dst, int o, int v, int64_t kpad,
@@ -652,10 +728,224 @@ __global__ __launch_bounds__(256) void wbuild_dma_kernel(WBuildArgs a) {
         }
 }
 
+// ------------------------------------------------------ W build, int8 slices
+// Exact-product mode (PI8). Every operand element is S = kI8Slices signed digits (pack_i8_kernel);
+// the product of two elements is sum_{t,u} d_t d'_u 128^-(t+u) times the two tensor scales. The
+// leading pairs t + u <= S + 1 are summed exactly on v_mfma_i32_16x16x64_i8 into one int32
+// accumulator per g = t + u; at the end of the f-terms and of the m-terms (each with its own scale)
+// the groups are converted and added to FP64 accumulators. Dropped pairs and the digits past S
+// bound the product error by about (S + 1) 128^-S of max|A| max|B| (2^-46 for S = 7).
+//
+// Work-group: 2 x 2 waves on a 64 x 64 tile (each wave 2 x 2 MFMA 16 x 16 blocks), chunks of 64 k
+// (all S slices) streamed into LDS by global_load_lds_dwordx4 through two stages. A row of a chunk
+// is S x 64 bytes, slice-major; the four 16-byte vectors of a slice are XOR-permuted by (r >> 2) & 3,
+// which puts the 16 rows of an MFMA fragment read on distinct banks (row pitch 448 bytes for S = 7).
+// The m-terms (K = o, padded to 32) are read as half-chunks: the two halves of a chunk may come
+// from two segments, the missing half of an odd count from a zero buffer.
+typedef int i4v __attribute__((ext_vector_type(4)));
+
+template <int S>
+__global__ __launch_bounds__(256) void wbuild_i8_kernel(WBuildArgs a) {
+    constexpr int BM = 64;
+    constexpr int RB = S * 64;     // bytes of one 64-deep chunk of a row
+    constexpr int OPB = BM * RB;   // bytes of one operand chunk
+    constexpr int NIW = S;         // wave loads per operand per chunk (OPB / 4 KiB)
+    static_assert(OPB == NIW * 4096, "operand chunk must split into whole wave loads");
+    __shared__ __attribute__((aligned(1024))) char lds[2][2 * OPB];
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = wave >> 1, wn = wave & 1;
+    const int lr = lane & 15, lq = lane >> 4;
+
+    const int nt2 = a.nt * a.nt;
+    const int gx = gridDim.x;
+    const int lin = xcd_remap(blockIdx.y * gx + blockIdx.x, gx * gridDim.y);
+    const int bat = lin / gx;
+    int bid = lin - bat * gx;
+    const bool fam1 = bid < a.ntile1;
+    if (!fam1)
+        bid -= a.ntile1;
+    const int64_t fx = bid / nt2;
+    const int tt = bid - (int)fx * nt2;
+    const int tr = tt / a.nt, tc = tt - tr * a.nt;
+    const int *q = a.ijk + 3 * bat;
+    const int64_t i = q[0], j = q[1], k = q[2];
+    const int v = a.v;
+    const int r0 = tr * BM, c0 = tc * BM;
+
+    const bool deg = a.halve && i == j;
+    if (deg && !fam1)
+        return;  // the whole work-group: W2 of this entry is not used
+    const int nfs = fam1 ? (deg ? 3 : 4) : 2;  // f segments (and as many m segments)
+    const int nfc = nfs * a.nchv;              // f chunks
+    const int nhalf = nfs * a.ncho;            // m half-chunks
+    const int total = nfc + (nhalf + 1) / 2;
+    auto fseg = [&](int n) { return fam1 && deg ? halved_seg(n) : n; };
+    auto mseg = [&](int n) { return (fam1 ? 4 : 2) + n; };
+
+    bool vr[2], vc[2];
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+        vr[b] = r0 + wm * 32 + b * 16 < v;
+        vc[b] = c0 + wn * 32 + b * 16 < v;
+    }
+
+    i4v acc[2][2][S];  // [x][y][g - 2]
+    double accd[2][2][4];
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++) {
+#pragma unroll
+            for (int g = 0; g < S; g++)
+                acc[x][y][g] = i4v{0, 0, 0, 0};
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                accd[x][y][r] = 0.0;
+        }
+
+    // this lane's LDS units: tile row, slice and logical 16-byte vector of the slice
+    int lrow[NIW], loff[NIW], lhalf[NIW], loffm[NIW];
+#pragma unroll
+    for (int u = 0; u < NIW; u++) {
+        const int qq = (wave * NIW + u) * 64 + lane;
+        const int r = qq / (4 * S), p = qq - r * (4 * S);
+        const int t = p >> 2, lv = (p & 3) ^ ((r >> 2) & 3);
+        lrow[u] = r;
+        loff[u] = t * 64 + lv * 16;
+        lhalf[u] = lv >> 1;
+        loffm[u] = t * 32 + (lv & 1) * 16;
+    }
+
+    const char *pa[NIW], *pb[NIW];  // f chunks: this lane's addresses at chunk 0 of the segment
+    int cur_fs = -1;
+    auto issue = [&](int c, int st) {
+        char *base = lds[st];
+        if (c < nfc) {
+            const int fs = c / a.nchv, kc = c - fs * a.nchv;
+            if (fs != cur_fs) {
+                cur_fs = fs;
+                const Seg sg = get_seg(a, fam1, fseg(fs), i, j, k, fx);
+#pragma unroll
+                for (int u = 0; u < NIW; u++) {
+                    const int gr = min(r0 + lrow[u], v - 1), gc = min(c0 + lrow[u], v - 1);
+                    pa[u] = sg.A + gr * sg.lda + loff[u];
+                    pb[u] = sg.B + gc * sg.ldb + loff[u];
+                }
+            }
+            const int64_t off = (int64_t)kc * RB;
+#pragma unroll
+            for (int u = 0; u < NIW; u++) {
+                __builtin_amdgcn_global_load_lds((const void *)(pa[u] + off),
+                                                 (lds_void *)(base + (wave * NIW + u) * 1024), 16, 0, 0);
+                __builtin_amdgcn_global_load_lds((const void *)(pb[u] + off),
+                                                 (lds_void *)(base + OPB + (wave * NIW + u) * 1024), 16, 0, 0);
+            }
+        } else {
+            const int mc = c - nfc;
+#pragma unroll
+            for (int u = 0; u < NIW; u++) {
+                const int h = 2 * mc + lhalf[u];
+                const char *sa = a.zero + loffm[u], *sb = a.zero + loffm[u];
+                if (h < nhalf) {
+                    const int ms = h / a.ncho, kh = h - ms * a.ncho;
+                    const Seg sg = get_seg(a, fam1, mseg(ms), i, j, k, fx);
+                    const int gr = min(r0 + lrow[u], v - 1), gc = min(c0 + lrow[u], v - 1);
+                    sa = sg.A + gr * sg.lda + (int64_t)kh * (S * 32) + loffm[u];
+                    sb = sg.B + gc * sg.ldb + (int64_t)kh * (S * 32) + loffm[u];
+                }
+                __builtin_amdgcn_global_load_lds((const void *)sa, (lds_void *)(base + (wave * NIW + u) * 1024), 16,
+                                                 0, 0);
+                __builtin_amdgcn_global_load_lds((const void *)sb,
+                                                 (lds_void *)(base + OPB + (wave * NIW + u) * 1024), 16, 0, 0);
+            }
+        }
+    };
+    // slice t, logical vector lv of tile row rr
+    auto frag = [&](const char *op, int rr, int t) {
+        return *(const i4v *)(op + rr * RB + t * 64 + ((lq ^ ((rr >> 2) & 3)) * 16));
+    };
+    auto compute = [&](int st) {
+        const char *As = lds[st];
+        const char *Bs = As + OPB;
+        i4v af[2][S], bf[2][S];
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int t = 0; t < S; t++) {
+                af[b][t] = frag(As, wm * 32 + b * 16 + lr, t);
+                bf[b][t] = frag(Bs, wn * 32 + b * 16 + lr, t);
+            }
+#pragma unroll
+        for (int g = 2; g <= S + 1; g++)
+#pragma unroll
+            for (int t = 1; t <= S; t++)
+                if (g - t >= 1 && g - t <= S)
+#pragma unroll
+                    for (int x = 0; x < 2; x++)
+                        if (vr[x])
+#pragma unroll
+                            for (int y = 0; y < 2; y++)
+                                if (vc[y])
+                                    acc[x][y][g - 2] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                                        af[x][t - 1], bf[y][g - t - 1], acc[x][y][g - 2], 0, 0, 0);
+    };
+    // accd += scale * sum_g acc_g 128^-g (smallest terms first); the int32 groups restart at 0
+    auto flush = [&](double scale) {
+#pragma unroll
+        for (int x = 0; x < 2; x++)
+#pragma unroll
+            for (int y = 0; y < 2; y++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    double sum = 0.0;
+#pragma unroll
+                    for (int g = S + 1; g >= 2; g--)
+                        sum += (double)acc[x][y][g - 2][r] * __builtin_ldexp(1.0, -7 * g);
+                    accd[x][y][r] = fma(scale, sum, accd[x][y][r]);
+#pragma unroll
+                    for (int g = 0; g < S; g++)
+                        acc[x][y][g][r] = 0;
+                }
+    };
+
+    issue(0, 0);
+    for (int c = 0; c < total; c++) {
+        wait_vm<0>();
+        __syncthreads();  // every wave's chunk c is in LDS; the other stage is free again
+        if (c + 1 < total)
+            issue(c + 1, (c + 1) & 1);
+        compute(c & 1);
+        if (c == nfc - 1)
+            flush(a.scale_f);
+    }
+    flush(a.scale_m);
+
+    double *W = (double *)(fam1 ? a.W1 : a.W2) + (int64_t)bat * a.wstride;
+    const int nbR = a.vR >> 3;
+    const int64_t fbase = ((int64_t)(fx >> 3) * nbR) * nbR * 512 + (fx & 7) * 64;
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++) {
+            const int col = c0 + wn * 32 + y * 16 + lr;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = r0 + wm * 32 + x * 16 + 4 * lq + r;
+                if (row < v && col < v)
+                    W[fbase + ((int64_t)(row >> 3) * nbR + (col >> 3)) * 512 + (row & 7) * 8 + (col & 7)] =
+                        accd[x][y][r];
+            }
+        }
+}
+
 // MFMA blocks per wave edge, from timing sweeps at v = 300: 2 (64x64 work-group tiles) for FP64
 // and FP32; 4 for BF16x3, whose three MFMAs per product need the larger register tile to keep
 // LDS reads per MFMA low.
 int wbuild_tile(int prec, int v) {
+    if (prec == PI8)
+        return 2;
     if (prec == PB3 && v >= 240)
         return 4;
     if (v > 32)
@@ -692,6 +982,11 @@ static hipError_t launch_wbuild_p(int blk, bool dma, const WBuildArgs &a, int nb
 }
 
 hipError_t launch_wbuild(int prec, int blk, bool dma, const WBuildArgs &a, int nbatch, hipStream_t s) {
+    if (prec == PI8) {  // one tile shape (64 x 64)
+        const dim3 grid((unsigned)(2 * a.ntile1), (unsigned)nbatch);
+        hipLaunchKernelGGL((wbuild_i8_kernel<kI8Slices>), grid, dim3(256), 0, s, a);
+        return hipGetLastError();
+    }
     if (prec == P64)
         return launch_wbuild_p<P64>(blk, dma, a, nbatch, s);
     if (prec == P32)
@@ -1031,7 +1326,7 @@ void orbit_table(const std::vector<int> &orbits, std::vector<uint64_t> &tab) {
 
 hipError_t launch_epilogue(int prec, const EpiArgs &a, int nbatch, hipStream_t s) {
     const dim3 grid((unsigned)a.ngrp, (unsigned)nbatch);
-    if (prec == P64)
+    if (prec == P64 || prec == PI8)  // both build W in FP64
         hipLaunchKernelGGL(epilogue_kernel<double>, grid, dim3(EPI_THREADS), 0, s, a);
     else
         hipLaunchKernelGGL(epilogue_kernel<float>, grid, dim3(EPI_THREADS), 0, s, a);

@@ -23,7 +23,7 @@ SCESS_ERR_OOM = -3
 SCESS_ERR_STATE = -4
 SCESS_CANCELLED = -5
 
-PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2, "ozaki": 3}
 
 _dp = ctypes.POINTER(ctypes.c_double)
 

@@ -54,6 +54,24 @@ def test_backward_matches_oracle(ctx, o, v, seed):
         assert err < 1e-11, (k, err)
 
 
+@pytest.mark.skip(reason="the ozaki W recompute in the backward is switched off (scess_t_backward rejects it): "
+                         "its run at o=34, v=7 faulted the GPU and no cause was found by reading or by CPU "
+                         "emulation of the W-build and back-contraction addresses")
+@pytest.mark.parametrize("o,v,seed", [(4, 9, 31), (34, 7, 32)])
+def test_backward_with_ozaki_w_matches_oracle(ctx, o, v, seed):
+    """W recomputed from int8 digit products (~2^-46 product error) in the backward: the gradients keep
+    1e-11 relative to the oracle; o = 34 spans two 32-wide occupied half-chunks."""
+    _say(f"case backward ozaki o={o} v={v}")
+    inp = make_triples_inputs(o, v, seed)
+    ctx.set_triples_inputs(inp, ("fp64", "ozaki"))
+    g = ctx.triples_backward("ozaki", dE=1.0)
+    go = oracle.t_grad(inp)
+    for k in NAMES:
+        scale = max(np.abs(go[k]).max(), 1e-300)
+        err = np.abs(g[k] - go[k]).max() / scale
+        assert err < 1e-11, (k, err)
+
+
 def test_backward_scales_with_dE_and_is_deterministic(ctx):
     _say("case backward dE scaling")
     inp = make_triples_inputs(4, 20, 5)

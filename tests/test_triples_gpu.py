@@ -46,6 +46,23 @@ def test_fp64_matches_oracle(ctx, o, v, seed):
     assert np.allclose(F, Fo, rtol=1e-11, atol=1e-13 * np.abs(Fo).max())
 
 
+@pytest.mark.parametrize("o,v,seed", [(2, 3, 21), (5, 19, 22), (6, 33, 23), (8, 40, 24), (7, 70, 25),
+                                      (3, 130, 26), (33, 9, 27), (40, 12, 28)])
+def test_ozaki_matches_oracle(ctx, o, v, seed):
+    """Int8-slice products are exact; W differs from FP64 by the dropped digit products
+    (~2^-46 of max|A| max|B| each), so E(T) keeps FP64's 1e-12 bar against the oracle. The sizes
+    cover v past one and two 64-deep chunks and o past one and two 32-wide half-chunks."""
+    _say(f"case ozaki o={o} v={v} seed={seed}")
+    inp = make_triples_inputs(o, v, seed)
+    ctx.set_triples_inputs(inp, ("fp64", "ozaki"))
+    E, F, _ = ctx.triples_forward("ozaki", with_F=True)
+    E64, F64, _ = ctx.triples_forward("fp64", with_F=True)
+    Eo = oracle.t_abc(inp)
+    _say(f"  E={E!r} Eo={Eo!r} rel={_rel(E, Eo):.3e} rel(fp64)={_rel(E64, Eo):.3e}")
+    assert _rel(E, Eo) < 1e-12, (E, Eo)
+    assert np.allclose(F, F64, rtol=1e-10, atol=1e-12 * np.abs(F64).max())
+
+
 def test_single_occupied_or_virtual_is_zero(ctx):
     for (o, v) in [(1, 9), (4, 1)]:
         _say(f"case zero o={o} v={v}")
@@ -58,8 +75,8 @@ def test_single_occupied_or_virtual_is_zero(ctx):
 def test_deterministic(ctx):
     _say("case deterministic o=6 v=40")
     inp = make_triples_inputs(6, 40, 10)
-    ctx.set_triples_inputs(inp, ("fp64", "fp32", "bf16x3"))
-    for p in ("fp64", "fp32", "bf16x3"):
+    ctx.set_triples_inputs(inp, ("fp64", "fp32", "bf16x3", "ozaki"))
+    for p in ("fp64", "fp32", "bf16x3", "ozaki"):
         E1, _ = ctx.triples_forward(p)
         ctx.set_batch(3)
         E2, _ = ctx.triples_forward(p)
