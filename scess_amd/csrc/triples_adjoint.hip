@@ -21,6 +21,7 @@
 namespace scess {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef double d2 __attribute__((ext_vector_type(2)));
 
 __device__ inline void nadd_a(double &s, double &c, double x) {
     const double t = s + x;
@@ -189,18 +190,19 @@ hipError_t launch_adjoint_g(int prec, const EpiArgs &a, double wgt, double *GW, 
 // pitch 80: the four 16-lane groups of a fragment read land 32 banks apart (conflict-free halves)
 constexpr int GT = 64, GKC = 16, GPITCH = 80;
 
-// One operand's 64 rows x 16 k of chunk (k1, k2c): 4 elements per thread, consecutive along the
-// operand's contiguous dimension so that the loads coalesce.
+// One operand's 64 rows x 16 k of chunk (k1, k2c): 4 elements per thread. Sixteen consecutive
+// lanes take 16 consecutive elements along the operand's contiguous dimension (k when the
+// operand is K-major, rows otherwise), so every load instruction reads 128-byte runs.
 struct TileLoad {
     bool kinner;
     __device__ TileLoad(const GemmOp &op) : kinner(op.s2 == 1 || (op.sr != 1 && (op.s2 < op.sr))) {}
     __device__ void coords(int tid, int u, int &rr, int &kk) const {
         if (kinner) {
-            rr = tid >> 2;
-            kk = (tid & 3) * 4 + u;
+            rr = (tid >> 4) + 16 * u;
+            kk = tid & 15;
         } else {
             kk = tid >> 4;
-            rr = (tid & 15) * 4 + u;
+            rr = (tid & 15) + 16 * u;
         }
     }
 };
@@ -226,10 +228,79 @@ __device__ inline void gemm_stash(const TileLoad &tl, const double *reg, double 
     }
 }
 
+// Paired loads for operands with a contiguous dimension and even strides (16-byte aligned pairs):
+// MODE 1 rows contiguous (sr == 1), a pair is rows (2j, 2j + 1) at one k, 32 lanes per k row;
+// MODE 2 K contiguous (s2 == 1), a pair is k (2j, 2j + 1) of one row, 8 lanes per row. Two pairs
+// per thread; reg[2u], reg[2u + 1] hold pair u. Elements past rext / K2 are zero.
+template <int MODE> __device__ inline void pair_coords(int tid, int u, int &rr, int &kk) {
+    const int p = tid + 256 * u;
+    if constexpr (MODE == 1) {
+        kk = p >> 5;
+        rr = (p & 31) * 2;
+    } else {
+        rr = p >> 3;
+        kk = (p & 7) * 2;
+    }
+}
+
+template <int MODE>
+__device__ inline void gemm_fetch_pair(const GemmOp &op, int64_t base, int r0, int rext, int k1, int k2c, int K2,
+                                       double *reg, int tid) {
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+        int rr, kk;
+        pair_coords<MODE>(tid, u, rr, kk);
+        const int gr = r0 + rr, gk = k2c * GKC + kk;
+        const bool full = MODE == 1 ? (gk < K2 && gr + 1 < rext) : (gr < rext && gk + 1 < K2);
+        const bool first = gr < rext && gk < K2;
+        const double *src = op.base + base + (int64_t)gr * op.sr + (int64_t)k1 * op.s1 + (int64_t)gk * op.s2;
+        if (full) {
+            const d2 x = *(const d2 *)src;
+            reg[2 * u] = x[0];
+            reg[2 * u + 1] = x[1];
+        } else {
+            reg[2 * u] = first ? src[0] : 0.0;
+            reg[2 * u + 1] = 0.0;
+        }
+    }
+}
+
+template <int MODE> __device__ inline void gemm_stash_pair(const double *reg, double *lds, int tid) {
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+        int rr, kk;
+        pair_coords<MODE>(tid, u, rr, kk);
+        if constexpr (MODE == 1) {
+            *(d2 *)&lds[kk * GPITCH + rr] = d2{reg[2 * u], reg[2 * u + 1]};
+        } else {
+            lds[kk * GPITCH + rr] = reg[2 * u];
+            lds[(kk + 1) * GPITCH + rr] = reg[2 * u + 1];
+        }
+    }
+}
+
+// One operand of chunk (k1, k2c) in load mode MODE (0 element-wise, 1 / 2 paired)
+template <int MODE>
+__device__ inline void fetch_op(const GemmOp &op, int64_t base, int r0, int rext, int k1, int k2c, int K2, double *reg,
+                                int tid) {
+    if constexpr (MODE == 0)
+        gemm_fetch(op, TileLoad(op), base, r0, rext, k1, k2c, K2, reg, tid);
+    else
+        gemm_fetch_pair<MODE>(op, base, r0, rext, k1, k2c, K2, reg, tid);
+}
+template <int MODE> __device__ inline void stash_op(const GemmOp &op, const double *reg, double *lds, int tid) {
+    if constexpr (MODE == 0)
+        gemm_stash(TileLoad(op), reg, lds, tid);
+    else
+        gemm_stash_pair<MODE>(reg, lds, tid);
+}
+
+// A0, A1, B0, B1: load modes of A and B in segments 0 and 1
+template <int A0, int A1, int B0, int B1>
 __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
     // two LDS buffers; the global loads of chunk c + 1 are in flight while chunk c is multiplied
-    __shared__ double As[2][GKC * GPITCH];
-    __shared__ double Bs[2][GKC * GPITCH];
+    __shared__ __attribute__((aligned(16))) double As[2][GKC * GPITCH];
+    __shared__ __attribute__((aligned(16))) double Bs[2][GKC * GPITCH];
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = wave >> 1, wn = wave & 1;
@@ -254,6 +325,13 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
 #pragma unroll
         for (int y = 0; y < 2; y++)
             acc[x][y] = d4{0.0, 0.0, 0.0, 0.0};
+    // MFMA blocks of this wave wholly past M or N are skipped (wave-uniform)
+    bool vm[2], vn[2];
+#pragma unroll
+    for (int x = 0; x < 2; x++) {
+        vm[x] = m0 + wm * 32 + x * 16 < g.M;
+        vn[x] = n0 + wn * 32 + x * 16 < g.N;
+    }
 
     double ra[4], rb[4];
     auto fetch = [&](int64_t ch) {
@@ -261,16 +339,26 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
         const int64_t cl = s == 0 ? ch : ch - nch[0];
         const GemmSeg &sg = g.seg[s];
         const int k1 = (int)(cl / nk2[s]), k2c = (int)(cl - (int64_t)k1 * nk2[s]);
-        gemm_fetch(sg.A, TileLoad(sg.A), (int64_t)beta * sg.A.sb, m0, g.M, k1, k2c, sg.K2, ra, tid);
-        gemm_fetch(sg.B, TileLoad(sg.B), (int64_t)beta * sg.B.sb, n0, g.N, k1, k2c, sg.K2, rb, tid);
+        const int64_t ba = (int64_t)beta * sg.A.sb, bb = (int64_t)beta * sg.B.sb;
+        if (s == 0) {
+            fetch_op<A0>(sg.A, ba, m0, g.M, k1, k2c, sg.K2, ra, tid);
+            fetch_op<B0>(sg.B, bb, n0, g.N, k1, k2c, sg.K2, rb, tid);
+        } else {
+            fetch_op<A1>(sg.A, ba, m0, g.M, k1, k2c, sg.K2, ra, tid);
+            fetch_op<B1>(sg.B, bb, n0, g.N, k1, k2c, sg.K2, rb, tid);
+        }
     };
     if (c_begin < c_end)
         fetch(c_begin);
     int buf = 0;
     for (int64_t ch = c_begin; ch < c_end; ch++) {
-        const GemmSeg &sg = g.seg[ch < nch[0] ? 0 : 1];
-        gemm_stash(TileLoad(sg.A), ra, As[buf], tid);
-        gemm_stash(TileLoad(sg.B), rb, Bs[buf], tid);
+        if (ch < nch[0]) {
+            stash_op<A0>(g.seg[0].A, ra, As[buf], tid);
+            stash_op<B0>(g.seg[0].B, rb, Bs[buf], tid);
+        } else {
+            stash_op<A1>(g.seg[1].A, ra, As[buf], tid);
+            stash_op<B1>(g.seg[1].B, rb, Bs[buf], tid);
+        }
         __syncthreads();
         if (ch + 1 < c_end)
             fetch(ch + 1);
@@ -286,9 +374,11 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
                 bf[y] = Bs[buf][kk * GPITCH + wn * 32 + y * 16 + (lane & 15)];
 #pragma unroll
             for (int x = 0; x < 2; x++)
+                if (vm[x])
 #pragma unroll
-                for (int y = 0; y < 2; y++)
-                    acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[x], bf[y], acc[x][y], 0, 0, 0);
+                    for (int y = 0; y < 2; y++)
+                        if (vn[y])
+                            acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[x], bf[y], acc[x][y], 0, 0, 0);
         }
         buf ^= 1;
     }
@@ -374,6 +464,13 @@ __global__ __launch_bounds__(256) void gemm_kc_kernel(GemmArgs g) {
 #pragma unroll
         for (int y = 0; y < BLK; y++)
             acc[x][y] = d4{0.0, 0.0, 0.0, 0.0};
+    // MFMA blocks of this wave wholly past M or N are skipped (wave-uniform)
+    bool vm[BLK], vn[BLK];
+#pragma unroll
+    for (int x = 0; x < BLK; x++) {
+        vm[x] = m0 + wm * BLK * 16 + x * 16 < g.M;
+        vn[x] = n0 + wn * BLK * 16 + x * 16 < g.N;
+    }
 
     // thread t loads vectors q = t + 256 u (u < NV) of the BM x 8 vector tile: row q / 8, vector q % 8
     d2v ra[NV], rb[NV];
@@ -449,9 +546,12 @@ __global__ __launch_bounds__(256) void gemm_kc_kernel(GemmArgs g) {
                 for (int t = 0; t < 2; t++)
 #pragma unroll
                     for (int x = 0; x < BLK; x++)
+                        if (vm[x])
 #pragma unroll
-                        for (int y = 0; y < BLK; y++)
-                            acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[x][t], bf[y][t], acc[x][y], 0, 0, 0);
+                            for (int y = 0; y < BLK; y++)
+                                if (vn[y])
+                                    acc[x][y] =
+                                        __builtin_amdgcn_mfma_f64_16x16x4f64(af[x][t], bf[y][t], acc[x][y], 0, 0, 0);
             }
         } else {
             // the lane's 4 k values of each block row, rounded (FP32) or split (hi + lo BF16)
@@ -477,9 +577,12 @@ __global__ __launch_bounds__(256) void gemm_kc_kernel(GemmArgs g) {
                 for (int t = 0; t < 4; t++)
 #pragma unroll
                     for (int x = 0; x < BLK; x++)
+                        if (vm[x])
 #pragma unroll
-                        for (int y = 0; y < BLK; y++)
-                            c32[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(a32[x][t], b32[y][t], c32[x][y], 0, 0, 0);
+                            for (int y = 0; y < BLK; y++)
+                                if (vn[y])
+                                    c32[x][y] =
+                                        __builtin_amdgcn_mfma_f32_16x16x4f32(a32[x][t], b32[y][t], c32[x][y], 0, 0, 0);
             } else {
                 s4v ah[BLK], al[BLK], bh[BLK], bl[BLK];
 #pragma unroll
@@ -495,6 +598,8 @@ __global__ __launch_bounds__(256) void gemm_kc_kernel(GemmArgs g) {
                 for (int x = 0; x < BLK; x++)
 #pragma unroll
                     for (int y = 0; y < BLK; y++) {
+                        if (!vm[x] || !vn[y])
+                            continue;
                         c32[x][y] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al[x], bh[y], c32[x][y], 0, 0, 0);
                         c32[x][y] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[x], bl[y], c32[x][y], 0, 0, 0);
                         c32[x][y] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[x], bh[y], c32[x][y], 0, 0, 0);
@@ -548,6 +653,17 @@ static int k_contiguous(const GemmArgs &g) {
     return r;
 }
 
+// Paired-load mode of one operand: 1 rows contiguous, 2 K contiguous (both with even strides and a
+// 16-byte aligned base), 0 otherwise
+static int pair_mode(const GemmOp &op) {
+    const bool even = !(op.s1 & 1) && !(op.sb & 1) && !(((uintptr_t)op.base) & 15);
+    if (even && op.sr == 1 && !(op.s2 & 1))
+        return 1;
+    if (even && op.s2 == 1 && !(op.sr & 1))
+        return 2;
+    return 0;
+}
+
 hipError_t launch_gemm(const GemmArgs &g, int batch, hipStream_t s) {
     if (const int kc = k_contiguous(g)) {
         // 64x64 tiles: for FP64 MFMA more resident work-groups beat larger register tiles
@@ -567,7 +683,19 @@ hipError_t launch_gemm(const GemmArgs &g, int batch, hipStream_t s) {
 #undef SCESS_KC
     } else {
         const int ntiles = ((g.M + GT - 1) / GT) * ((g.N + GT - 1) / GT);
-        hipLaunchKernelGGL(gemm_kernel, dim3((unsigned)ntiles, (unsigned)batch, (unsigned)g.splits), dim3(256), 0, s, g);
+        const dim3 grid((unsigned)ntiles, (unsigned)batch, (unsigned)g.splits);
+        int a0 = pair_mode(g.seg[0].A), b0 = pair_mode(g.seg[0].B);
+        int a1 = g.nseg > 1 ? pair_mode(g.seg[1].A) : a0, b1 = g.nseg > 1 ? pair_mode(g.seg[1].B) : b0;
+        if (!a0 || !a1 || !b0 || !b1 || b0 != b1)
+            a0 = a1 = b0 = b1 = 0;  // element-wise loads for every operand
+#define SCESS_GP(A0, A1, B)                                                                       \
+    if (a0 == A0 && a1 == A1 && b0 == B) {                                                        \
+        hipLaunchKernelGGL((gemm_kernel<A0, A1, B, B>), grid, dim3(256), 0, s, g);                \
+    } else
+        SCESS_GP(1, 1, 1) SCESS_GP(1, 2, 1) SCESS_GP(2, 1, 1) SCESS_GP(2, 2, 1)
+        SCESS_GP(1, 1, 2) SCESS_GP(1, 2, 2) SCESS_GP(2, 1, 2) SCESS_GP(2, 2, 2)
+        hipLaunchKernelGGL((gemm_kernel<0, 0, 0, 0>), grid, dim3(256), 0, s, g);
+#undef SCESS_GP
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || g.splits == 1)
