@@ -305,9 +305,13 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = wave >> 1, wn = wave & 1;
     const int ntn = (g.N + GT - 1) / GT;
-    const int tm = blockIdx.x / ntn, tn = blockIdx.x - tm * ntn;
+    // tiles of one (batch, split) on one XCD, so the tiles sharing an operand panel share its L2
+    const int nx = gridDim.x, ny = gridDim.y;
+    const int lin = xcd_remap(blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z), nx * ny * gridDim.z);
+    const int bx = lin % nx, byz = lin / nx;
+    const int tm = bx / ntn, tn = bx - tm * ntn;
     const int m0 = tm * GT, n0 = tn * GT;
-    const int beta = blockIdx.y, split = blockIdx.z;
+    const int beta = byz % ny, split = byz / ny;
 
     // flat chunk index over the segments; this split takes a fixed contiguous range of it
     int64_t nch[2] = {0, 0};
@@ -408,9 +412,21 @@ __global__ __launch_bounds__(256) void gemm_reduce_kernel(GemmArgs g) {
     if (idx >= (int64_t)g.M * g.N)
         return;
     const int m = (int)(idx / g.N), n = (int)(idx - (int64_t)m * g.N);
+    const int64_t mn = (int64_t)g.M * g.N;
+    const double *src = g.scratch + idx;
     double s = 0.0;
-    for (int p = 0; p < g.splits; p++)
-        s += g.scratch[(int64_t)p * g.M * g.N + idx];
+    int p = 0;
+    for (; p + 8 <= g.splits; p += 8) {  // eight loads in flight, added in split order
+        double x[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            x[u] = src[(int64_t)(p + u) * mn];
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            s += x[u];
+    }
+    for (; p < g.splits; p++)
+        s += src[(int64_t)p * mn];
     g.C[(int64_t)m * g.sCm + (int64_t)n * g.sCn] += g.alpha * s;
 }
 
@@ -445,9 +461,13 @@ __global__ __launch_bounds__(256) void gemm_kc_kernel(GemmArgs g) {
     const int wm = wave >> 1, wn = wave & 1;
     const int lr = lane & 15, lq = lane >> 4;
     const int ntn = (g.N + BM - 1) / BM;
-    const int tm = blockIdx.x / ntn, tn = blockIdx.x - tm * ntn;
+    // tiles of one (batch, split) on one XCD, so the tiles sharing an operand panel share its L2
+    const int nx = gridDim.x, ny = gridDim.y;
+    const int lin = xcd_remap(blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z), nx * ny * gridDim.z);
+    const int bx = lin % nx, byz = lin / nx;
+    const int tm = bx / ntn, tn = bx - tm * ntn;
     const int m0 = tm * BM, n0 = tn * BM;
-    const int beta = blockIdx.y, split = blockIdx.z;
+    const int beta = byz % ny, split = byz / ny;
 
     int64_t nch[2] = {0, 0};
     int nk2[2] = {1, 1};

@@ -23,6 +23,15 @@ constexpr int kElemBytes[kNumPrec] = {8, 4, 4, kI8Slices};
 
 inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
+// Work-group id -> logical id so that consecutive logical ids run on the same
+// XCD: the dispatcher hands work-groups to the 8 XCDs of the MI355X round-robin,
+// and neighbouring work items share operand rows (one L2 per XCD).
+__device__ inline int xcd_remap(int w, int n) {
+    const int xcd = w & 7, idx = w >> 3;
+    const int per = n >> 3, rem = n & 7;
+    return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+}
+
 // Device-resident operands of the W contractions in one precision.
 //   X [p][x][y][f]  = (px|yf)         o * v * v * vK
 //   T [q][r][z][f]  = t_qr^zf         o * o * v * vK

@@ -43,14 +43,6 @@ __device__ inline uint16_t bf16_rne(float f) {
 }
 __device__ inline float bf16_to_float(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
 
-// Work-group id -> logical id so that consecutive logical ids run on the same
-// XCD: the dispatcher hands work-groups to the 8 XCDs of the MI355X round-robin,
-// and neighbouring work items share operand rows and cache lines of W.
-__device__ inline int xcd_remap(int w, int n) {
-    const int xcd = w & 7, idx = w >> 3;
-    const int per = n >> 3, rem = n & 7;
-    return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
-}
 
 // Neumaier compensated accumulation
 __device__ inline void nadd(double &s, double &c, double x) {
