@@ -314,14 +314,11 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
     const int beta = byz % ny, split = byz / ny;
 
     // flat chunk index over the segments; this split takes a fixed contiguous range of it
-    int64_t nch[2] = {0, 0};
-    int nk2[2] = {1, 1};
-    for (int s = 0; s < g.nseg; s++) {
-        nk2[s] = (g.seg[s].K2 + GKC - 1) / GKC;
-        nch[s] = (int64_t)g.seg[s].K1 * nk2[s];
-    }
-    const int64_t total = nch[0] + nch[1];
-    const int64_t c_begin = total * split / g.splits, c_end = total * (split + 1) / g.splits;
+    // (segment values as scalars: a runtime-indexed local array would live in scratch memory)
+    const int nk20 = (g.seg[0].K2 + GKC - 1) / GKC, nk21 = g.nseg > 1 ? (g.seg[1].K2 + GKC - 1) / GKC : 1;
+    const int nch0 = g.seg[0].K1 * nk20, nch1 = g.nseg > 1 ? g.seg[1].K1 * nk21 : 0;
+    const int total = nch0 + nch1;
+    const int c_begin = (int)((int64_t)total * split / g.splits), c_end = (int)((int64_t)total * (split + 1) / g.splits);
 
     d4 acc[2][2];
 #pragma unroll
@@ -337,26 +334,24 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
         vn[x] = n0 + wn * 32 + x * 16 < g.N;
     }
 
-    double ra[4], rb[4];
-    auto fetch = [&](int64_t ch) {
-        const int s = ch < nch[0] ? 0 : 1;
-        const int64_t cl = s == 0 ? ch : ch - nch[0];
-        const GemmSeg &sg = g.seg[s];
-        const int k1 = (int)(cl / nk2[s]), k2c = (int)(cl - (int64_t)k1 * nk2[s]);
-        const int64_t ba = (int64_t)beta * sg.A.sb, bb = (int64_t)beta * sg.B.sb;
-        if (s == 0) {
-            fetch_op<A0>(sg.A, ba, m0, g.M, k1, k2c, sg.K2, ra, tid);
-            fetch_op<B0>(sg.B, bb, n0, g.N, k1, k2c, sg.K2, rb, tid);
+    // two register sets: chunk c + 2 is loaded while chunk c is multiplied (c + 1 waits in the other)
+    double ra0[4], rb0[4], ra1[4], rb1[4];
+    auto fetch = [&](int ch, double *ra, double *rb) {
+        if (ch < nch0) {
+            const GemmSeg &sg = g.seg[0];
+            const int k1 = ch / nk20, k2c = ch - k1 * nk20;
+            fetch_op<A0>(sg.A, (int64_t)beta * sg.A.sb, m0, g.M, k1, k2c, sg.K2, ra, tid);
+            fetch_op<B0>(sg.B, (int64_t)beta * sg.B.sb, n0, g.N, k1, k2c, sg.K2, rb, tid);
         } else {
-            fetch_op<A1>(sg.A, ba, m0, g.M, k1, k2c, sg.K2, ra, tid);
-            fetch_op<B1>(sg.B, bb, n0, g.N, k1, k2c, sg.K2, rb, tid);
+            const GemmSeg &sg = g.seg[1];
+            const int cl = ch - nch0, k1 = cl / nk21, k2c = cl - k1 * nk21;
+            fetch_op<A1>(sg.A, (int64_t)beta * sg.A.sb, m0, g.M, k1, k2c, sg.K2, ra, tid);
+            fetch_op<B1>(sg.B, (int64_t)beta * sg.B.sb, n0, g.N, k1, k2c, sg.K2, rb, tid);
         }
     };
-    if (c_begin < c_end)
-        fetch(c_begin);
     int buf = 0;
-    for (int64_t ch = c_begin; ch < c_end; ch++) {
-        if (ch < nch[0]) {
+    auto step = [&](int ch, double *ra, double *rb) {
+        if (ch < nch0) {
             stash_op<A0>(g.seg[0].A, ra, As[buf], tid);
             stash_op<B0>(g.seg[0].B, rb, Bs[buf], tid);
         } else {
@@ -364,8 +359,8 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
             stash_op<B1>(g.seg[1].B, rb, Bs[buf], tid);
         }
         __syncthreads();
-        if (ch + 1 < c_end)
-            fetch(ch + 1);
+        if (ch + 2 < c_end)
+            fetch(ch + 2, ra, rb);
 #pragma unroll
         for (int t = 0; t < 4; t++) {
             const int kk = t * 4 + (lane >> 4);
@@ -385,6 +380,15 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
                             acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[x], bf[y], acc[x][y], 0, 0, 0);
         }
         buf ^= 1;
+    };
+    if (c_begin < c_end)
+        fetch(c_begin, ra0, rb0);
+    if (c_begin + 1 < c_end)
+        fetch(c_begin + 1, ra1, rb1);
+    for (int ch = c_begin; ch < c_end; ch += 2) {
+        step(ch, ra0, rb0);
+        if (ch + 1 < c_end)
+            step(ch + 1, ra1, rb1);
     }
 
 #pragma unroll
@@ -469,14 +473,11 @@ __global__ __launch_bounds__(256) void gemm_kc_kernel(GemmArgs g) {
     const int m0 = tm * BM, n0 = tn * BM;
     const int beta = byz % ny, split = byz / ny;
 
-    int64_t nch[2] = {0, 0};
-    int nk2[2] = {1, 1};
-    for (int s = 0; s < g.nseg; s++) {
-        nk2[s] = (g.seg[s].K2 + 15) / 16;
-        nch[s] = (int64_t)g.seg[s].K1 * nk2[s];
-    }
-    const int64_t total = nch[0] + nch[1];
-    const int64_t c_begin = total * split / g.splits, c_end = total * (split + 1) / g.splits;
+    // segment values as scalars: a runtime-indexed local array would live in scratch memory
+    const int nk20 = (g.seg[0].K2 + 15) / 16, nk21 = g.nseg > 1 ? (g.seg[1].K2 + 15) / 16 : 1;
+    const int nch0 = g.seg[0].K1 * nk20, nch1 = g.nseg > 1 ? g.seg[1].K1 * nk21 : 0;
+    const int total = nch0 + nch1;
+    const int c_begin = (int)((int64_t)total * split / g.splits), c_end = (int)((int64_t)total * (split + 1) / g.splits);
 
     d4 acc[BLK][BLK];
 #pragma unroll
@@ -494,11 +495,11 @@ __global__ __launch_bounds__(256) void gemm_kc_kernel(GemmArgs g) {
 
     // thread t loads vectors q = t + 256 u (u < NV) of the BM x 8 vector tile: row q / 8, vector q % 8
     d2v ra[NV], rb[NV];
-    auto fetch = [&](int64_t ch) {
-        const int s = ch < nch[0] ? 0 : 1;
-        const int64_t cl = s == 0 ? ch : ch - nch[0];
-        const GemmSeg &sg = g.seg[s];
-        const int k1 = (int)(cl / nk2[s]), k0 = (int)(cl - (int64_t)k1 * nk2[s]) * 16;
+    auto fetch = [&](int ch) {
+        const bool s0 = ch < nch0;
+        const int cl = s0 ? ch : ch - nch0, nk = s0 ? nk20 : nk21;
+        const GemmSeg &sg = s0 ? g.seg[0] : g.seg[1];
+        const int k1 = cl / nk, k0 = (cl - k1 * nk) * 16;
         const double *pa = sg.A.base + (int64_t)beta * sg.A.sb + (int64_t)k1 * sg.A.s1;
         const double *pb = sg.B.base + (int64_t)beta * sg.B.sb + (int64_t)k1 * sg.B.s1;
 #pragma unroll
@@ -545,7 +546,7 @@ __global__ __launch_bounds__(256) void gemm_kc_kernel(GemmArgs g) {
     if (c_begin < c_end)
         fetch(c_begin);
     int buf = 0;
-    for (int64_t ch = c_begin; ch < c_end; ch++) {
+    for (int ch = c_begin; ch < c_end; ch++) {
         stash(buf);
         __syncthreads();
         if (ch + 1 < c_end)
@@ -685,6 +686,13 @@ static int pair_mode(const GemmOp &op) {
 }
 
 hipError_t launch_gemm(const GemmArgs &g, int batch, hipStream_t s) {
+    {  // the kernels index K chunks (16 deep) with 32-bit integers
+        int64_t chunks = 0;
+        for (int q = 0; q < g.nseg; q++)
+            chunks += (int64_t)g.seg[q].K1 * ((g.seg[q].K2 + 15) / 16);
+        if (chunks >= (int64_t)1 << 31 || g.nseg < 1 || g.nseg > 2)
+            return hipErrorInvalidValue;
+    }
     if (const int kc = k_contiguous(g)) {
         // 64x64 tiles: for FP64 MFMA more resident work-groups beat larger register tiles
         int blk = 2;
