@@ -41,8 +41,9 @@ def ctx():
     c.close()
 
 
-@pytest.mark.parametrize("o,v,seed", [(2, 3, 1), (3, 5, 2), (4, 9, 3), (5, 17, 4)])
+@pytest.mark.parametrize("o,v,seed", [(2, 3, 1), (3, 5, 2), (4, 9, 3), (5, 17, 4), (3, 70, 5)])
 def test_backward_matches_oracle(ctx, o, v, seed):
+    """(3, 70): v past one 64-wide tile of the back-contraction GEMMs."""
     _say(f"case backward o={o} v={v}")
     inp = make_triples_inputs(o, v, seed)
     ctx.set_triples_inputs(inp, ("fp64",))
