@@ -14,7 +14,9 @@
 
 #include <array>
 #include <cstdint>
+#include <map>
 #include <sstream>
+#include <string>
 #include <vector>
 
 namespace SCESS {
@@ -105,6 +107,43 @@ RCCSD_Wfn RCCSD_T_FromMO(const TEITensor &eri, const std::vector<double> &orbita
 
 }  // namespace CC
 
+// src/Quantum/Molecule.h: atoms (positions in bohr) and the reference's built-in molecules
+// (src/Quantum/Molecule.cpp:194-231, 281-294).
+struct Atom {
+    std::string symbol;
+    int Z = 0;
+    double position[3] = {0.0, 0.0, 0.0};
+};
+class Molecule {
+public:
+    std::vector<Atom> atoms;
+    int NumElectrons() const;
+    double NuclearRepulsion() const;
+    static Molecule GenerateWaterMolecule();
+    static Molecule GenerateHydrogenMolecule();
+    static Molecule GenerateLithiumHydrideMolecule();
+    static Molecule GenerateAmmoniaMolecule();
+    // "H2O", "H2", "LiH", "NH3": the names the reference's command line accepts (Application.cpp:105-110)
+    static Molecule FromName(const std::string &name);
+};
+
+namespace Basis {
+// One contracted shell of an element (src/Quantum/Basis/BasisSet.cpp:139-181): spherical unless the BSE
+// shell is "gto_cartesian"; coefficients of the normalised primitives, as in the BSE file.
+struct ShellData {
+    int l = 0;
+    bool pure = false;
+    std::vector<double> exponents, coefficients;
+};
+struct BasisSet {
+    std::string name;
+    std::map<int, std::vector<ShellData>> elements;  // by atomic number
+};
+// A BSE JSON file: a path, or a basis set name looked up (case-insensitively, "6-31G*" = "6-31G(d)") as
+// <name>.json in $SCESS_BASIS_DIR or its subdirectories (the layout of the reference's Assets/Basis).
+BasisSet LoadBasisSet(const std::string &nameOrPath);
+}  // namespace Basis
+
 namespace HF {
 // The fields of HF_Wfn (src/Quantum/HF/HF.h:26-67) that the closed-shell CC methods read.
 // C_Alpha is nao x nmo ColMajor (column p = MO p), as Eigen::MatrixXd.
@@ -119,13 +158,27 @@ struct HF_Wfn {
 
 namespace Integrals {
 using TEITensor = Tensor<4>;
-// The AO two-electron tensor of MolInts (src/Quantum/Integrals/Integrals.h:28-69), chemists' (mu nu|kap lam).
+// MolInts (src/Quantum/Integrals/Integrals.h:28-69): overlap S, kinetic T, nuclear attraction V and the
+// AO two-electron tensor, chemists' (mu nu|kap lam).
 struct MolInts {
+    Tensor<2> S, T, V;
     TEITensor eri;
 };
+// EvaluateMolecularIntegrals (src/Quantum/Integrals/Integrals.cpp:28-80) for shells up to f, on the host.
+MolInts EvaluateMolecularIntegrals(const Molecule &molecule, const Basis::BasisSet &basis);
 // TransformTEI (src/Quantum/Integrals/IntegralTransforms.cpp:29-59) on the GPU.
 TEITensor TransformTEI(const TEITensor &tei, const std::vector<double> &C, int64_t nmo, bool convertChemToPhys = false);
 }  // namespace Integrals
+
+namespace HF {
+// RHF (src/Quantum/HF/HF.h:80-86, RHF.cpp:34-211): closed-shell SCF from the core-Hamiltonian guess with
+// symmetric orthogonalisation and Pulay DIIS (6 Fock matrices); converged when the energy changes by
+// less than convergeLimit and the rms of X^T (FDS - SDF) X is below sqrt(convergeLimit). Throws when it
+// does not converge; returns an empty HF_Wfn when the thread asks to stop.
+HF_Wfn RHF(const Molecule &molecule, const Basis::BasisSet &basis, double convergeLimit = 1e-5,
+           size_t scfMaxIterations = 100, Integrals::MolInts *mintsOut = nullptr, const Thread *thread = nullptr,
+           std::stringstream *console = nullptr);
+}  // namespace HF
 
 namespace CC {
 // RCCSD_T (src/Quantum/CC/CC.h:160-164, RCCSD_T.cpp:170-195): nOcc = leading occupied orbitals of

@@ -15,7 +15,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libscess_gpu.so")
 CLI = os.path.join(HERE, "scess_cc")
-SOURCES = ["triples_kernels.hip", "triples_adjoint.hip", "ccsd_engine.hip", "scess_capi.hip", "cc_api.cpp"]
+SOURCES = ["triples_kernels.hip", "triples_adjoint.hip", "ccsd_engine.hip", "scess_capi.hip", "cc_api.cpp",
+           "scf_host.cpp"]
 ARCH = "gfx950"
 
 # Packed FP32 instructions are disabled for all device code.

@@ -1,13 +1,14 @@
-// scess_cc: command-line runner of the closed-shell CC methods on an FCIDUMP file.
+// scess_cc: command-line runner of the closed-shell CC methods.
 //
-//   scess_cc ccsd   <FCIDUMP>
-//   scess_cc ccsd_t <FCIDUMP>
-//   scess_cc hf     <FCIDUMP>   (no GPU: E_HF and the orbital energies from the dump)
+//   scess_cc rhf|ccsd|ccsd_t <molecule> <basis>   as the reference's `SCESS ccsd_t H2O cc-pVDZ`
+//   scess_cc ccsd|ccsd_t|hf  <FCIDUMP>            from canonical RHF molecular orbitals
 //
-// The reference's CLI (src/SCESS/Application.cpp:23-277, Driver.cpp:203-231) builds the
-// molecule, basis, integrals and RHF itself; this runner starts from canonical RHF molecular
-// orbitals instead, given as an FCIDUMP (NORB, NELEC, one-electron h_pq, chemists' (pq|rs) with
-// 8-fold symmetry, core energy), and prints the reference's console output of the CC driver.
+// The first form follows the reference's CLI (src/SCESS/Application.cpp:23-277) and drivers
+// (Driver.cpp:51-65, 173-231): a built-in molecule (H2, LiH, NH3, H2O), a BSE-JSON basis set found
+// by name under $SCESS_BASIS_DIR (the reference's Assets/Basis layout), AO integrals and RHF on the
+// host, then the MO transform, CCSD and (T) on the GPU; `rhf` needs no GPU. The second form starts
+// from an FCIDUMP (NORB, NELEC, one-electron h_pq, chemists' (pq|rs) with 8-fold symmetry, core
+// energy) and prints the reference's console output of the CC driver.
 // Orbital energies and E_HF follow from the dump: e_p = h_pp + sum_k [2(pp|kk) - (pk|kp)],
 // E_HF = E_core + sum_k (h_kk + e_k). Exit code 0 on convergence, 1 otherwise (as the reference).
 #include "../../include/scess_cc.h"
@@ -18,6 +19,7 @@
 #include <fstream>
 #include <iomanip>
 #include <iostream>
+#include <map>
 #include <string>
 
 namespace {
@@ -78,12 +80,71 @@ Dump read_fcidump(const std::string &path) {
     return d;
 }
 
+// Driver.cpp:51-65 (rhf), 173-200 (ccsd), 203-231 (ccsd_t) with a built-in molecule and a basis set
+int run_molecule(const std::string &method, const std::string &molName, const std::string &basisName) {
+    static const std::map<std::string, std::string> longName = {
+        {"H2", "Hydrogen (H2)"}, {"LiH", "Lithium Hydride (LiH)"}, {"H2O", "Water (H2O)"}, {"NH3", "Ammonia (NH3)"}};
+    const SCESS::Molecule mol = SCESS::Molecule::FromName(molName);
+    const SCESS::Basis::BasisSet basis = SCESS::Basis::LoadBasisSet(basisName);
+    const std::string title = longName.at(molName);
+    const char *what = method == "rhf" ? "RHF" : method == "ccsd" ? "CCSD" : "CCSD(T)";
+    std::cout << "--- Performing " << what << " calculation on " << title << " ---\n";
+    std::cout << std::setprecision(12);
+    SCESS::Integrals::MolInts mints;
+    std::stringstream console;
+    // tighter than the reference driver's 1e-8, so that the CC energies carry no SCF residue
+    const SCESS::HF::HF_Wfn hf = SCESS::HF::RHF(mol, basis, 1e-10, 100, &mints, nullptr, &console);
+    std::cout << console.str() << "\n";
+    console.str("");
+    if (method == "rhf") {
+        std::cout << "\n\n--- Restricted Hartree-Fock Single Point Energy ---\n  Molecule = " << title
+                  << "\n  E(RHF) = " << hf.E_HF << " hartrees\n";
+        return 0;
+    }
+    SCESS::CC::RCCSD_Wfn cc;
+    if (method == "ccsd_t") {
+        cc = SCESS::CC::RCCSD_T(hf, mints, nullptr, &console);
+    } else {
+        int64_t nocc = 0;
+        while (nocc < (int64_t)hf.Occ_Alpha.size() && hf.Occ_Alpha[(size_t)nocc])
+            nocc++;
+        const SCESS::CC::TEITensor eri = SCESS::Integrals::TransformTEI(mints.eri, hf.C_Alpha, hf.nmo, true);
+        cc = SCESS::CC::SolveRCCSD_DIIS(eri, hf.Ep_Alpha, nocc, nullptr, nullptr, &console);
+    }
+    std::cout << console.str() << "\n";
+    if (method == "ccsd_t")
+        std::cout << "\n\n--- CCSD(T) Single Point Energy ---\n  Molecule = " << title << "\n  E(RHF) = " << hf.E_HF
+                  << " hartrees\n  E(CCSD(T) correlation) = " << cc.E_CCSD + cc.E_T
+                  << " hartrees\n  E(Total) = " << cc.E_CCSD + hf.E_HF + cc.E_T << " hartrees\n";
+    else
+        std::cout << "\n\n--- Coupled-Cluster Singles & Doubles Single Point Energy ---\n  Molecule = " << title
+                  << "\n  E(RHF) = " << hf.E_HF << " hartrees\n  E(CCSD correlation) = " << cc.E_CCSD
+                  << " hartrees\n  E(Total) = " << cc.E_CCSD + hf.E_HF << " hartrees\n";
+    return cc.converged ? 0 : 1;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
+    if (argc == 4) {
+        std::string method = argv[1];
+        for (char &c : method)
+            c = (char)std::tolower((unsigned char)c);
+        if (method != "rhf" && method != "ccsd" && method != "ccsd_t") {
+            std::fprintf(stderr, "%s is not a valid method (rhf, ccsd, ccsd_t)\n", argv[1]);
+            return 1;
+        }
+        try {
+            return run_molecule(method, argv[2], argv[3]);
+        } catch (const std::exception &ex) {
+            std::fprintf(stderr, "scess_cc: %s\n", ex.what());
+            return 1;
+        }
+    }
     const std::string method = argc == 3 ? argv[1] : "";
     if (method != "ccsd" && method != "ccsd_t" && method != "hf") {
-        std::fprintf(stderr, "usage: %s ccsd|ccsd_t|hf <FCIDUMP>\n", argv[0]);
+        std::fprintf(stderr, "usage: %s rhf|ccsd|ccsd_t <molecule> <basis>\n       %s ccsd|ccsd_t|hf <FCIDUMP>\n",
+                     argv[0], argv[0]);
         return 1;
     }
     const bool triples = method == "ccsd_t";

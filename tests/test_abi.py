@@ -45,5 +45,7 @@ def test_library_exports_the_cxx_method_api():
         pytest.skip("libscess_gpu.so not built or nm missing")
     out = subprocess.run(["nm", "-DC", "--defined-only", gpu.LIB_PATH], capture_output=True, text=True).stdout
     for n in ("SCESS::CC::SolveRCCSD_DIIS(", "SCESS::CC::RCCSD_TriplesCorrection(", "SCESS::CC::RCCSD_T(",
-              "SCESS::CC::RCCSD_T_FromMO(", "SCESS::Integrals::TransformTEI("):
+              "SCESS::CC::RCCSD_T_FromMO(", "SCESS::Integrals::TransformTEI(", "SCESS::HF::RHF(",
+              "SCESS::Integrals::EvaluateMolecularIntegrals(", "SCESS::Basis::LoadBasisSet(",
+              "SCESS::Molecule::GenerateWaterMolecule("):
         assert n in out, n

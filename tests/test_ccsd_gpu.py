@@ -176,6 +176,37 @@ def test_cli_ccsd_t_water(tmp_path, basis):
     assert "Turning on DIIS" in r.stdout and "CCSD procedure converged" in r.stdout
 
 
+@pytest.mark.parametrize("method,basis", [("ccsd_t", "cc-pVDZ"), ("ccsd", "STO-3G")])
+def test_cli_builtin_water(tmp_path, method, basis):
+    """`scess_cc ccsd_t H2O cc-pVDZ` as the reference's command line: host AO integrals and RHF, then the MO
+    transform, CCSD and (T) on the GPU, against the published energies (Tests.cpp:41-53)."""
+    import os
+    import re
+    import subprocess
+    from test_cli import make_basis_dir
+    _say(f"case cli builtin water {method} {basis}")
+    cli = os.path.join(os.path.dirname(gpu.LIB_PATH), "scess_cc")
+    ref = {"STO-3G": (-74.9629832288, -0.0494026324, -0.0000673499, 1e-6),
+           "cc-pVDZ": (-76.0267859418, -0.2133077024, -0.0030574132, 1e-8)}[basis]
+    env = dict(os.environ, SCESS_BASIS_DIR=make_basis_dir(tmp_path))
+    r = subprocess.run([cli, method, "H2O", basis], capture_output=True, text=True, timeout=120, env=env)
+    _say(r.stdout[-700:], r.stderr[-600:])
+    assert r.returncode == 0
+    num = r"(-?[0-9.]+(?:e[-+]?\d+)?)"
+    ehf = float(re.search(r"E\(RHF\) = " + num, r.stdout).group(1))
+    assert abs(ehf - ref[0]) < ref[3]
+    if method == "ccsd_t":
+        corr = float(re.search(r"E\(CCSD\(T\) correlation\) = " + num, r.stdout).group(1))
+        assert abs(corr - (ref[1] + ref[2])) < 2e-7
+        et = float(re.search(r"E\(T\) = " + num, r.stdout).group(1))
+        assert abs(et - ref[2]) < 1e-7
+    else:
+        corr = float(re.search(r"E\(CCSD correlation\) = " + num, r.stdout).group(1))
+        assert abs(corr - ref[1]) < 1e-7
+    total = float(re.search(r"E\(Total\) = " + num + " hartrees\n*$", r.stdout).group(1))
+    assert abs(total - (ehf + corr)) < 1e-9
+
+
 def test_water_baseline_geometry_gpu_vs_oracle(ctx):
     """Water at O-H 0.9572 A, H-O-H 104.52 deg, cc-pVDZ: CCSD and (T) on the GPU against the CPU
     oracles (1e-10 Eh); no published energy exists for this geometry."""

@@ -38,3 +38,64 @@ def test_cli_rejects_bad_usage(tmp_path):
     assert subprocess.run([CLI, "fci", "x"], capture_output=True).returncode == 1
     r = subprocess.run([CLI, "hf", str(tmp_path / "missing")], capture_output=True, text=True)
     assert r.returncode == 1 and "cannot open" in r.stderr
+
+
+GOLDEN_BASIS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "basis_h_o.json")
+
+
+def make_basis_dir(root):
+    """The reference's Assets/Basis layout (family folders of BSE JSON files) with the H and O entries
+    of cc-pVDZ and STO-3G from the golden fixture."""
+    import json
+    data = json.load(open(GOLDEN_BASIS))
+    for fam, name in (("Dunning", "cc-pVDZ"), ("Pople", "STO-3G")):
+        os.makedirs(os.path.join(root, fam), exist_ok=True)
+        with open(os.path.join(root, fam, name + ".json"), "w") as f:
+            json.dump({"name": name, "elements": data[name]}, f)
+    return str(root)
+
+
+def _rhf(args, basis_dir):
+    env = dict(os.environ, SCESS_BASIS_DIR=basis_dir)
+    return subprocess.run([CLI, *args], capture_output=True, text=True, timeout=120, env=env)
+
+
+@pytest.mark.skipif(not os.path.exists(CLI), reason="scess_cc not built")
+@pytest.mark.parametrize("mol,basis,ref,tol", [("H2O", "cc-pVDZ", -76.0267859418, 1e-8),
+                                                ("H2O", "STO-3G", -74.9629832288, 1e-6),
+                                                ("H2", "STO-3G", -1.1175058851, 1e-8)])
+def test_cli_rhf_builtin_molecule(tmp_path, mol, basis, ref, tol):
+    """`scess_cc rhf <molecule> <basis>`: the host AO integrals and RHF against the published RHF energies
+    (src/SCESS/Tests.cpp:36-53, same tolerances as the oracle chain) and, for water, the oracle's own
+    integrals + RHF (1e-9)."""
+    r = _rhf(["rhf", mol, basis], make_basis_dir(tmp_path))
+    assert r.returncode == 0, r.stderr
+    e = float(re.search(r"E\(RHF\) = (-?[0-9.]+)", r.stdout).group(1))
+    assert abs(e - ref) < tol
+    if mol == "H2O":
+        from oracle import scf
+        atoms = scf.water_builtin()
+        S, H, G = scf.ao_integrals(basis, atoms)
+        E_HF = scf.rhf(S, H, G, 5, scf.nuclear_repulsion(atoms))[0]
+        assert abs(e - E_HF) < 1e-9
+
+
+@pytest.mark.skipif(not os.path.exists(CLI), reason="scess_cc not built")
+def test_cli_basis_lookup_and_errors(tmp_path):
+    d = make_basis_dir(tmp_path)
+    ok = _rhf(["RHF", "H2O", "cc-pvdz"], d)  # method and basis names are case-insensitive
+    assert ok.returncode == 0, ok.stderr
+    path = os.path.join(d, "Pople", "STO-3G.json")
+    by_path = _rhf(["rhf", "H2", path], d)  # a file path instead of a name
+    assert by_path.returncode == 0 and "E(RHF) = -1.11750588" in by_path.stdout
+    r = _rhf(["rhf", "H2O", "def2-TZVP"], d)
+    assert r.returncode == 1 and "not an available basis set" in r.stderr
+    r = _rhf(["rhf", "CH4", "STO-3G"], d)
+    assert r.returncode == 1 and "not an available molecule" in r.stderr
+    r = _rhf(["rhf", "LiH", "STO-3G"], d)  # the fixture holds H and O only
+    assert r.returncode == 1 and "no entry for Li" in r.stderr
+    r = _rhf(["mp2", "H2O", "STO-3G"], d)
+    assert r.returncode == 1 and "not a valid method" in r.stderr
+    env = {k: v for k, v in os.environ.items() if k != "SCESS_BASIS_DIR"}
+    r = subprocess.run([CLI, "rhf", "H2O", "STO-3G"], capture_output=True, text=True, env=env)
+    assert r.returncode == 1 and "SCESS_BASIS_DIR" in r.stderr
