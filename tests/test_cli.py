@@ -48,7 +48,7 @@ def make_basis_dir(root):
     of cc-pVDZ and STO-3G from the golden fixture."""
     import json
     data = json.load(open(GOLDEN_BASIS))
-    for fam, name in (("Dunning", "cc-pVDZ"), ("Pople", "STO-3G")):
+    for fam, name in (("Dunning", "cc-pVDZ"), ("Pople", "STO-3G"), ("Pople", "6-31G")):
         os.makedirs(os.path.join(root, fam), exist_ok=True)
         with open(os.path.join(root, fam, name + ".json"), "w") as f:
             json.dump({"name": name, "elements": data[name]}, f)
@@ -63,6 +63,7 @@ def _rhf(args, basis_dir):
 @pytest.mark.skipif(not os.path.exists(CLI), reason="scess_cc not built")
 @pytest.mark.parametrize("mol,basis,ref,tol", [("H2O", "cc-pVDZ", -76.0267859418, 1e-8),
                                                 ("H2O", "STO-3G", -74.9629832288, 1e-6),
+                                                ("H2O", "6-31G", -75.9839769556, 1e-8),
                                                 ("H2", "STO-3G", -1.1175058851, 1e-8)])
 def test_cli_rhf_builtin_molecule(tmp_path, mol, basis, ref, tol):
     """`scess_cc rhf <molecule> <basis>`: the host AO integrals and RHF against the published RHF energies
@@ -72,7 +73,7 @@ def test_cli_rhf_builtin_molecule(tmp_path, mol, basis, ref, tol):
     assert r.returncode == 0, r.stderr
     e = float(re.search(r"E\(RHF\) = (-?[0-9.]+)", r.stdout).group(1))
     assert abs(e - ref) < tol
-    if mol == "H2O":
+    if mol == "H2O" and basis != "6-31G":  # the oracle chain is pinned on the cc-pVDZ and STO-3G entries
         from oracle import scf
         atoms = scf.water_builtin()
         S, H, G = scf.ao_integrals(basis, atoms)
