@@ -481,9 +481,9 @@ int scess_t_forward_ex(scess_ctx *c, int prec, double *e_t, double *F_ijk, doubl
     const int64_t wstride = vR * vR * vR;  // W in 8x8x8 blocks over vR^3
 
     // batch size: a multiple of 8 occupied triples per launch (8 for 8-byte W, 16 for 4-byte W), so that
-    // the XCD-ordered work-groups give each of the 8 XCDs whole triples and their operand slices stay in
-    // that XCD's L2 (27% faster FP64 W-build at o=50, v=500 than one triple per launch); fewer when the
-    // W sets (2 arrays x 2 sets x nbatch x vR^3) exceed a quarter of the free device memory
+    // the XCD-ordered work-groups give each of the 8 XCDs whole triples (measured: the FP64 W-build at
+    // o=50, v=500 is 27% faster than with one triple per launch, at the same HBM bytes per triple);
+    // fewer when the W sets (2 arrays x 2 sets x nbatch x vR^3) exceed a quarter of the free device memory
     int64_t nbatch = c->nbatch_req;
     if (nbatch <= 0) {
         const double per = 4.0 * (double)wstride * (double)accb;
