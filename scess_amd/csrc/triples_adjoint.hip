@@ -454,12 +454,16 @@ __device__ inline short bf16_bits(float f) {  // round to nearest even
 }
 __device__ inline float bf16_value(short h) { return __uint_as_float(((uint32_t)(uint16_t)h) << 16); }
 
-template <int BLK, bool VEC, int PREC>
+// DMA: operand chunks go from global memory straight into LDS (global_load_lds_dwordx4, each lane fetching
+// the logical vector its swizzled LDS slot holds), as in the W-build; vectors reaching past K2 load the
+// row's first vector instead and are rewritten (value or zero) after the chunk has landed.
+template <int BLK, bool VEC, int PREC, bool DMA>
 __global__ __launch_bounds__(256) void gemm_kc_kernel(GemmArgs g) {
     constexpr int BM = 2 * BLK * 16;   // tile rows (and columns)
     constexpr int RB = 128;            // bytes of one 16-double K chunk of a row
     constexpr int NV = BM * 8 / 256;   // 16-byte vectors per thread per operand per chunk
-    __shared__ __attribute__((aligned(16))) char lds[2][2 * BM * RB];
+    static_assert(!DMA || (VEC && BM * RB % 4096 == 0), "direct-to-LDS loads need 16-byte vectors, whole KiB");
+    __shared__ __attribute__((aligned(1024))) char lds[2][2 * BM * RB];
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = wave >> 1, wn = wave & 1;
@@ -543,14 +547,7 @@ __global__ __launch_bounds__(256) void gemm_kc_kernel(GemmArgs g) {
         }
     };
 
-    if (c_begin < c_end)
-        fetch(c_begin);
-    int buf = 0;
-    for (int ch = c_begin; ch < c_end; ch++) {
-        stash(buf);
-        __syncthreads();
-        if (ch + 1 < c_end)
-            fetch(ch + 1);
+    auto compute = [&](int buf) {
         const char *As = lds[buf], *Bs = As + BM * RB;
         // lane (lr, lq) holds k = 4 lq .. 4 lq + 3 of its row: vectors 2 lq and 2 lq + 1
         if constexpr (PREC == P64) {
@@ -634,7 +631,90 @@ __global__ __launch_bounds__(256) void gemm_kc_kernel(GemmArgs g) {
                     for (int r = 0; r < 4; r++)
                         acc[x][y][r] += (double)c32[x][y][r];
         }
-        buf ^= 1;
+    };
+
+    int buf = 0;
+    if constexpr (DMA) {
+        constexpr int OPB = BM * RB, NIW = OPB / 4096;
+        // this lane's DMA units: tile row and the first k (even) of the logical vector its LDS slot holds
+        int lrow[NIW], lk[NIW];
+#pragma unroll
+        for (int u = 0; u < NIW; u++) {
+            const int qq = (wave * NIW + u) * 64 + lane, r = qq >> 3;
+            lrow[u] = r;
+            lk[u] = 2 * ((qq & 7) ^ kswz(r));
+        }
+        auto chunk_pos = [&](int ch, const GemmSeg *&sg, int &k1, int &k0) {
+            const bool s0 = ch < nch0;
+            const int cl = s0 ? ch : ch - nch0, nk = s0 ? nk20 : nk21;
+            sg = s0 ? &g.seg[0] : &g.seg[1];
+            k1 = cl / nk;
+            k0 = (cl - k1 * nk) * 16;
+        };
+        auto issue = [&](int ch, int st) {
+            const GemmSeg *sg;
+            int k1, k0;
+            chunk_pos(ch, sg, k1, k0);
+            const double *pa = sg->A.base + (int64_t)beta * sg->A.sb + (int64_t)k1 * sg->A.s1;
+            const double *pb = sg->B.base + (int64_t)beta * sg->B.sb + (int64_t)k1 * sg->B.s1;
+            char *base = lds[st];
+#pragma unroll
+            for (int u = 0; u < NIW; u++) {
+                const int k = k0 + lk[u], kk = k + 1 < sg->K2 ? k : 0;  // inside the row, else its first vector
+                const int gm = min(m0 + lrow[u], g.M - 1), gn = min(n0 + lrow[u], g.N - 1);
+                __builtin_amdgcn_global_load_lds((const void *)(pa + (int64_t)gm * sg->A.sr + kk),
+                                                 (lds_void *)(base + (wave * NIW + u) * 1024), 16, 0, 0);
+                __builtin_amdgcn_global_load_lds((const void *)(pb + (int64_t)gn * sg->B.sr + kk),
+                                                 (lds_void *)(base + OPB + (wave * NIW + u) * 1024), 16, 0, 0);
+            }
+        };
+        // after the chunk has landed: the vectors reaching past K2 get (x[k], 0) or (0, 0)
+        auto fixup = [&](int ch, int st) {
+            const GemmSeg *sg;
+            int k1, k0;
+            chunk_pos(ch, sg, k1, k0);
+            if (k0 + 16 <= sg->K2)
+                return false;
+            const double *pa = sg->A.base + (int64_t)beta * sg->A.sb + (int64_t)k1 * sg->A.s1;
+            const double *pb = sg->B.base + (int64_t)beta * sg->B.sb + (int64_t)k1 * sg->B.s1;
+            char *base = lds[st];
+#pragma unroll
+            for (int u = 0; u < NIW; u++) {
+                const int k = k0 + lk[u];
+                if (k + 1 < sg->K2)
+                    continue;
+                const int gm = min(m0 + lrow[u], g.M - 1), gn = min(n0 + lrow[u], g.N - 1);
+                const bool one = k < sg->K2;
+                *(d2v *)(base + (wave * NIW + u) * 1024 + lane * 16) =
+                    d2v{one ? pa[(int64_t)gm * sg->A.sr + k] : 0.0, 0.0};
+                *(d2v *)(base + OPB + (wave * NIW + u) * 1024 + lane * 16) =
+                    d2v{one ? pb[(int64_t)gn * sg->B.sr + k] : 0.0, 0.0};
+            }
+            return true;  // wave-uniform: every lane of the block sees the same chunk
+        };
+        if (c_begin < c_end)
+            issue(c_begin, 0);
+        for (int ch = c_begin; ch < c_end; ch++) {
+            wait_vm<0>();
+            __syncthreads();  // chunk ch is in LDS; the other stage is free again
+            if (fixup(ch, buf))
+                __syncthreads();
+            if (ch + 1 < c_end)
+                issue(ch + 1, buf ^ 1);
+            compute(buf);
+            buf ^= 1;
+        }
+    } else {
+        if (c_begin < c_end)
+            fetch(c_begin);
+        for (int ch = c_begin; ch < c_end; ch++) {
+            stash(buf);
+            __syncthreads();
+            if (ch + 1 < c_end)
+                fetch(ch + 1);
+            compute(buf);
+            buf ^= 1;
+        }
     }
 
 #pragma unroll
@@ -701,12 +781,21 @@ hipError_t launch_gemm(const GemmArgs &g, int batch, hipStream_t s) {
         const int bm = 32 * blk;
         const int nt = ((g.M + bm - 1) / bm) * ((g.N + bm - 1) / bm);
         const dim3 grid((unsigned)nt, (unsigned)batch, (unsigned)g.splits);
-#define SCESS_KC(B, V, P)                                                                         \
-    if (blk == B && (kc == 2) == V && g.prec == P) {                                              \
-        hipLaunchKernelGGL((gemm_kc_kernel<B, V, P>), grid, dim3(256), 0, s, g);                  \
+        // direct-to-LDS loads when the rows are 16-byte vectors and at least two elements long (the first
+        // vector of every row is then inside it); SCESS_GEMM_DMA=0 keeps the register-staged loads
+        bool dma = kc == 2 && blk == 2;
+        for (int q = 0; q < g.nseg; q++)
+            dma = dma && g.seg[q].K2 >= 2 && g.M >= 1 && g.N >= 1;
+        if (const char *e = std::getenv("SCESS_GEMM_DMA"))
+            dma = dma && std::atoi(e) != 0;
+#define SCESS_KC(B, V, P, D)                                                                      \
+    if (blk == B && (kc == 2) == V && g.prec == P && dma == D) {                                  \
+        hipLaunchKernelGGL((gemm_kc_kernel<B, V, P, D>), grid, dim3(256), 0, s, g);               \
     } else
-        SCESS_KC(2, true, P64) SCESS_KC(2, false, P64) SCESS_KC(2, true, P32) SCESS_KC(2, false, P32)
-        SCESS_KC(2, true, PB3) SCESS_KC(2, false, PB3) SCESS_KC(4, true, P64) SCESS_KC(4, false, P64)
+        SCESS_KC(2, true, P64, true) SCESS_KC(2, true, P32, true) SCESS_KC(2, true, PB3, true)
+        SCESS_KC(2, true, P64, false) SCESS_KC(2, false, P64, false) SCESS_KC(2, true, P32, false)
+        SCESS_KC(2, false, P32, false) SCESS_KC(2, true, PB3, false) SCESS_KC(2, false, PB3, false)
+        SCESS_KC(4, true, P64, false) SCESS_KC(4, false, P64, false)
         return hipErrorInvalidValue;
 #undef SCESS_KC
     } else {

@@ -23,6 +23,15 @@ constexpr int kElemBytes[kNumPrec] = {8, 4, 4, kI8Slices};
 
 inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
+// LDS pointer type of global_load_lds (direct-to-LDS loads)
+typedef __attribute__((address_space(3))) void lds_void;
+
+// s_waitcnt vmcnt(N) leaving the other counters alone (gfx9 encoding)
+template <int N> __device__ inline void wait_vm() {
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
 // Work-group id -> logical id so that consecutive logical ids run on the same
 // XCD: the dispatcher hands work-groups to the 8 XCDs of the MI355X round-robin,
 // and neighbouring work items share operand rows (one L2 per XCD).

@@ -509,7 +509,6 @@ __global__ __launch_bounds__(256) void wbuild_kernel(WBuildArgs a) {
 // buffers, so no registers hold staged data and no ds_write is issued. Rows are
 // stored unpadded; the 16-byte vectors of row r are permuted by swz(r) so that
 // the MFMA fragment reads (16 rows x 16 bytes per pass) hit distinct banks.
-typedef __attribute__((address_space(3))) void lds_void;
 
 template <int VPR> __device__ inline int lds_swz(int r) {
     if constexpr (VPR == 8)
@@ -518,11 +517,6 @@ template <int VPR> __device__ inline int lds_swz(int r) {
         return ((r >> 1) ^ (r >> 3)) & 3;
 }
 
-// s_waitcnt vmcnt(N) leaving the other counters alone (gfx9 encoding)
-template <int N> __device__ inline void wait_vm() {
-    static_assert(N >= 0 && N < 64, "vmcnt range");
-    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
-}
 
 template <int PREC, int BLK, int STAGES>
 __global__ __launch_bounds__(256) void wbuild_dma_kernel(WBuildArgs a) {
