@@ -180,6 +180,12 @@ def adjoint_bench(args):
                        "abs_over_max": abs(fd - an) / gmax[name]})
         log(f"[adjoint] fd {n_}: d{name}{tuple(int(x) for x in ix)} analytic {an:.12e} fd {fd:.12e} rel {rel:.2e}")
     ctx.close()
+    cpu = None
+    if not args.no_cpu_baseline and nshards == 1:
+        try:
+            cpu = adjoint_cpu_baseline(args.seed, wprec)
+        except Exception as e:  # reported, never fatal
+            cpu = {"error": repr(e)}
     line = {
         "metric": "(T) adjoint seconds per backward and central-difference check",
         "value": sec, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -193,8 +199,36 @@ def adjoint_bench(args):
         "fd": {"h": h, "entries": len(checks), "max_rel_err": worst,
                "max_abs_err_over_max_grad": max(c["abs_over_max"] for c in checks) if checks else None,
                "n_rel_err_le_1e-6": sum(c["rel_err"] <= 1e-6 for c in checks), "checks": checks},
+        "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def adjoint_cpu_baseline(seed: int, wprec: str, o: int = 6, v: int = 40):
+    """The oracle's FP64 backward of E(T) (C, one thread per occupied triple block) timed at a small size on
+    this host, next to the GPU backward at the same size; the reference has no adjoint."""
+    import numpy as np
+
+    import oracle
+    from scess_amd import gpu
+    from scess_amd.synthetic import make_triples_inputs
+
+    inp = make_triples_inputs(o, v, seed)
+    t = time.perf_counter()
+    oracle.t_grad(inp)
+    cpu_s = time.perf_counter() - t
+    ctx = gpu.Context(0)
+    ctx.set_triples_inputs(inp, tuple(dict.fromkeys(("fp64", wprec))))
+    ctx.triples_backward(wprec)
+    gpu_t = []
+    for _ in range(3):
+        t = time.perf_counter()
+        ctx.triples_backward(wprec)
+        gpu_t.append(time.perf_counter() - t)
+    ctx.close()
+    return {"value": cpu_s, "unit": "s per backward", "cores": oracle.num_threads(), "kind": "port",
+            "sample": f"one backward at o={o}, v={v} (C oracle of the adjoint; the reference has none)",
+            "gpu_same_size_s": float(np.median(gpu_t))}
 
 
 def ccsd_bench(args):
@@ -230,6 +264,12 @@ def ccsd_bench(args):
     E1 = ctx.ccsd_energy(n1, n2)
     ctx.close()
     sec = sum(times) / len(times)
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = ccsd_cpu_baseline(args.seed, v_bench=v)
+        except Exception as e:  # reported, never fatal
+            cpu = {"error": repr(e)}
     ladder = 2.0 * o * o * float(v) ** 4
     line = {
         "metric": "CCSD amplitude update seconds (Cholesky-regenerated ladder)",
@@ -243,8 +283,53 @@ def ccsd_bench(args):
         "E_MP2": E0, "E_after_one_update": E1,
         "ladder_flops": ladder, "ladder_regeneration_flops": 2.0 * float(v) ** 4 * ci.naux,
         "tflops_reference_formulation": (ladder + 16.0 * o ** 3 * float(v) ** 3 + 4.0 * o * float(v) ** 4) / sec / 1e12,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def ccsd_cpu_baseline(seed: int, v_bench: int, o: int = 8, v: int = 64):
+    """The oracle's UpdateAmplitudes (numpy restatement of RCCSD.cpp:101-237, with the v^4 (ac|bd) block
+    as the reference forms it) timed at a small size on this host, next to the GPU update at the same
+    size. At the benchmarked size (o=50, v=500) the reference formulation needs the 500 GB v^4 tensor and
+    is not timed."""
+    import numpy as np
+
+    from oracle import ccsd_oracle as co
+    from scess_amd import gpu
+    from scess_amd.synthetic import full_eri_chem, make_ccsd_inputs, make_triples_inputs, t2_from_ovov
+
+    inp = make_triples_inputs(o, v, seed, keep_factor=True)
+    eri = full_eri_chem(inp.L)
+    ints = co.make_blocks(eri, o)
+    eia, eijab = co.denominators(inp.e, o)
+    t1 = np.zeros((o, v))
+    t2 = np.ascontiguousarray(inp.t2)
+    cpu_t = []
+    for _ in range(3):  # median of 3, or fewer when one update already takes seconds
+        t = time.perf_counter()
+        co.update_amplitudes(ints, eia, eijab, None, t1, t2)
+        cpu_t.append(time.perf_counter() - t)
+        if sum(cpu_t) > 10.0:
+            break
+    ci = make_ccsd_inputs(o, v, seed)
+    ctx = gpu.Context(0)
+    ctx.set_ccsd_inputs(o, v, ci.e, ci.blocks, chol_vv=ci.chol_vv)
+    t1f = np.zeros((o, v), order="F")
+    t2f = t2_from_ovov(ci.blocks["ovov"], ci.e, o)
+    ctx.ccsd_update(t1f, t2f)
+    gpu_t = []
+    for _ in range(3):
+        t = time.perf_counter()
+        ctx.ccsd_update(t1f, t2f)
+        gpu_t.append(time.perf_counter() - t)
+    ctx.close()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    return {"value": float(np.median(cpu_t)), "unit": "s per update", "cores": threads, "kind": "port",
+            "sample": f"one update at o={o}, v={v} (numpy oracle, BLAS threads = {threads}); the reference "
+                      f"formulation at the benchmarked size needs the {8.0 * v_bench ** 4 / 1e9:.0f} GB v^4 tensor "
+                      "and is not run",
+            "gpu_same_size_s": float(np.median(gpu_t))}
 
 
 def main():
