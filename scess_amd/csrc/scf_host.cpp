@@ -1023,21 +1023,20 @@ HF_Wfn RHF(const Molecule &molecule, const Basis::BasisSet &basis, double conver
                     B[(size_t)i * m + j] = s;
                     bmax = std::max(bmax, std::fabs(s));
                 }
-            if (bmax < 1e-28)  // the commutator vanishes (e.g. by symmetry): nothing to extrapolate
-                goto diag;
-            for (int i = 0; i < m - 1; i++)
-                for (int j = 0; j < m - 1; j++)
-                    B[(size_t)i * m + j] /= bmax;
-            for (int i = 0; i < m - 1; i++)
-                B[(size_t)i * m + m - 1] = B[(size_t)(m - 1) * m + i] = -1.0;
-            r[m - 1] = -1.0;
-            const std::vector<double> cdiis = solve(B, r, m);
-            std::fill(F.begin(), F.end(), 0.0);
-            for (int i = 0; i < m - 1; i++)
-                for (size_t k = 0; k < F.size(); k++)
-                    F[k] += cdiis[i] * fs[i][k];
+            if (bmax >= 1e-28) {  // else the commutator vanishes (e.g. by symmetry): nothing to extrapolate
+                for (int i = 0; i < m - 1; i++)
+                    for (int j = 0; j < m - 1; j++)
+                        B[(size_t)i * m + j] /= bmax;
+                for (int i = 0; i < m - 1; i++)
+                    B[(size_t)i * m + m - 1] = B[(size_t)(m - 1) * m + i] = -1.0;
+                r[m - 1] = -1.0;
+                const std::vector<double> cdiis = solve(B, r, m);
+                std::fill(F.begin(), F.end(), 0.0);
+                for (int i = 0; i < m - 1; i++)
+                    for (size_t k = 0; k < F.size(); k++)
+                        F[k] += cdiis[i] * fs[i][k];
+            }
         }
-    diag:
         diagonalise(F, eps, C);
         D = density(C);
     }
