@@ -826,6 +826,22 @@ hipError_t launch_gemm(const GemmArgs &g, int batch, hipStream_t s) {
 // V(a,b,c) = U0[a][b] t_k^c + U1[a][c] t_j^b + U2[b][c] t_i^a (U rows pitched vR, t1 rows vR):
 //   dU0[a][b] = sum_c G_V t_k^c,  dU1[a][c] = sum_b G_V t_j^b,  dU2[b][c] = sum_a G_V t_i^a
 //   dt_k^c = sum_ab G_V U0[a][b], dt_j^b = sum_ac G_V U1[a][c], dt_i^a = sum_bc G_V U2[b][c]
+// sum_z f(z), z = 0..n-1, in order, with eight loads in flight
+template <typename F> __device__ inline void nsum8(int n, F f, double &s, double &c) {
+    int z = 0;
+    for (; z + 8 <= n; z += 8) {
+        double x[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+            x[q] = f(z + q);
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+            nadd_a(s, c, x[q]);
+    }
+    for (; z < n; z++)
+        nadd_a(s, c, f(z));
+}
+
 __global__ __launch_bounds__(256) void vback_du_kernel(const double *GV, const double *t1, int i, int j, int k, int v,
                                                         int vR, double *dU) {
     const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -840,16 +856,13 @@ __global__ __launch_bounds__(256) void vback_du_kernel(const double *GV, const d
     if (w == 0) {  // dU0[x=a][y=b]
         const double *g = GV + x * R2 + (int64_t)y * vR;
         const double *t = t1 + (int64_t)k * vR;
-        for (int z = 0; z < v; z++)
-            nadd_a(s, c, g[z] * t[z]);
+        nsum8(v, [&](int z) { return g[z] * t[z]; }, s, c);
     } else if (w == 1) {  // dU1[x=a][y=c]
         const double *t = t1 + (int64_t)j * vR;
-        for (int z = 0; z < v; z++)
-            nadd_a(s, c, GV[x * R2 + (int64_t)z * vR + y] * t[z]);
+        nsum8(v, [&](int z) { return GV[x * R2 + (int64_t)z * vR + y] * t[z]; }, s, c);
     } else {  // dU2[x=b][y=c]
         const double *t = t1 + (int64_t)i * vR;
-        for (int z = 0; z < v; z++)
-            nadd_a(s, c, GV[z * R2 + (int64_t)x * vR + y] * t[z]);
+        nsum8(v, [&](int z) { return GV[z * R2 + (int64_t)x * vR + y] * t[z]; }, s, c);
     }
     dU[idx] = s + c;
 }
@@ -860,20 +873,32 @@ __global__ __launch_bounds__(256) void vback_dt_kernel(const double *GV, const d
     const int64_t R2 = (int64_t)vR * vR, vr = (int64_t)v * vR;
     const int64_t n = (int64_t)v * v;
     double s = 0.0, c = 0.0;
-    for (int64_t t = threadIdx.x; t < n; t += 256) {
-        const int p = (int)(t / v), q = (int)(t - (int64_t)p * v);
-        double g, u;
-        if (w == 0) {  // dt_k[x=c] = sum_ab G[a][b][c] U0[a][b]
-            g = GV[p * R2 + (int64_t)q * vR + x];
-            u = U[(int64_t)p * vR + q];
-        } else if (w == 1) {  // dt_j[x=b] = sum_ac G[a][b][c] U1[a][c]
-            g = GV[p * R2 + (int64_t)x * vR + q];
-            u = U[vr + (int64_t)p * vR + q];
-        } else {  // dt_i[x=a] = sum_bc G[a][b][c] U2[b][c]
-            g = GV[x * R2 + (int64_t)p * vR + q];
-            u = U[2 * vr + (int64_t)p * vR + q];
+    // terms t = p v + q: dt_k[x=c] = sum_ab G[a][b][c] U0[a][b] (w = 0), dt_j[x=b] = sum_ac G[a][b][c]
+    // U1[a][c] (w = 1), dt_i[x=a] = sum_bc G[a][b][c] U2[b][c] (w = 2); this thread takes t = tid, tid + 256,
+    // ... in order, eight loads in flight, (p, q) advanced without divisions
+    const double *Uw = U + (int64_t)w * vr;
+    int p = (int)(threadIdx.x / v), q = (int)(threadIdx.x % v);
+    for (int64_t t0 = threadIdx.x; t0 < n; t0 += 256 * 8) {
+        double xs[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            double term = 0.0;  // past the end: adds nothing
+            if (t0 + 256 * u < n) {
+                const double g = w == 0   ? GV[p * R2 + (int64_t)q * vR + x]
+                                 : w == 1 ? GV[p * R2 + (int64_t)x * vR + q]
+                                          : GV[x * R2 + (int64_t)p * vR + q];
+                term = g * Uw[(int64_t)p * vR + q];
+            }
+            xs[u] = term;
+            q += 256;
+            while (q >= v) {
+                q -= v;
+                p++;
+            }
         }
-        nadd_a(s, c, g * u);
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            nadd_a(s, c, xs[u]);
     }
     rs[threadIdx.x] = s;
     rc[threadIdx.x] = c;
