@@ -1,8 +1,8 @@
 """ctypes bindings to the CPU oracle (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and bench.py's
-cpu_baseline leg use this module as the checker and as the reported CPU
-baseline. The product path never imports it.
+cpu_baseline leg use this module to check the GPU results and as the reported
+CPU baseline. The product path never imports it.
 """
 
 from __future__ import annotations
