@@ -836,18 +836,25 @@ int scess_t_backward(scess_ctx *c, int prec, double dE, scess_t_grad *grad) {
             he = launch_vpieces(t.S, dijk, (int)o, (int)v, (int)vR, U, 1, c->stream);
         if (he == hipSuccess)
             he = launch_adjoint_g(prec, ea, dE * 2.0 / deg, GW, GV, c->stream);
+        // With occ[0] == occ[1] (every triple with two equal indices, as evaluated) G is symmetric in its
+        // first two indices, and the permutation terms pair up: the term with the values 0 and 1 of its
+        // permutation exchanged (PERM rows 0-2, 1-3, 4-5) reads the same operands and writes the same
+        // output. Such a triple runs one term of each pair with alpha = 2 (half the GEMM work).
+        const bool twin = occ[0] == occ[1];
         // dX[p][x][y][f] += sum_z G(x,y,z) T[r][q][z][f]: the two permutation terms with the same
         // position for x (PERM rows 2m, 2m+1) write the same dX[p], so they run as one GEMM with
-        // two K segments
+        // two K segments (twin: pair 1 equals pair 0, and the two segments of pair 2 are equal)
         for (int pr = 0; pr < 3 && he == hipSuccess; pr++) {
+            if (twin && pr == 1)
+                continue;
             const int64_t p = occ[PERM[2 * pr][0]], sx = pos_stride[PERM[2 * pr][0]];
             GemmArgs g{};
-            g.nseg = 2;
+            g.nseg = twin && pr == 2 ? 1 : 2;
             g.M = (int)v;
             g.N = (int)v;
-            g.alpha = 1.0;
+            g.alpha = twin ? 2.0 : 1.0;
             g.splits = 1;
-            for (int h = 0; h < 2; h++) {
+            for (int h = 0; h < g.nseg; h++) {
                 const int sgm = 2 * pr + h;
                 const int64_t q = occ[PERM[sgm][1]], r = occ[PERM[sgm][2]];
                 const int64_t sy = pos_stride[PERM[sgm][1]], sz = pos_stride[PERM[sgm][2]];
@@ -864,6 +871,9 @@ int scess_t_backward(scess_ctx *c, int prec, double dE, scess_t_grad *grad) {
             he = launch_gemm(g, (int)v, c->stream);
         }
         for (int sgm = 0; sgm < 6 && he == hipSuccess; sgm++) {
+            if (twin && (sgm == 2 || sgm == 3 || sgm == 5))
+                continue;  // the twins of terms 0, 1, 4
+            const double alpha = twin ? 2.0 : 1.0;
             const int64_t p = occ[PERM[sgm][0]], q = occ[PERM[sgm][1]], r = occ[PERM[sgm][2]];
             const int64_t sx = pos_stride[PERM[sgm][0]], sy = pos_stride[PERM[sgm][1]], sz = pos_stride[PERM[sgm][2]];
             // dT[r][q][z][f] += sum_{x,y} G(x,y,z) X[p][x][y][f]
@@ -872,7 +882,7 @@ int scess_t_backward(scess_ctx *c, int prec, double dE, scess_t_grad *grad) {
                 g.nseg = 1;
                 g.M = (int)v;
                 g.N = (int)v;
-                g.alpha = 1.0;
+                g.alpha = alpha;
                 g.splits = PT;
                 g.scratch = scr;
                 GemmSeg &sg = g.seg[0];
@@ -896,7 +906,7 @@ int scess_t_backward(scess_ctx *c, int prec, double dE, scess_t_grad *grad) {
                 g.nseg = 1;
                 g.M = (int)v;
                 g.N = (int)o;
-                g.alpha = 1.0;
+                g.alpha = alpha;
                 g.splits = PR;
                 g.scratch = scr;
                 GemmSeg &sg = g.seg[0];
@@ -918,16 +928,19 @@ int scess_t_backward(scess_ctx *c, int prec, double dE, scess_t_grad *grad) {
         // dTM[r][y][z][m] += sum_x Rn[p][q][x][m] G(x,y,z): the two terms with the same position for z
         // write the same dTM[r]; one GEMM with two K segments each (rows z and batch y when z is the
         // contiguous position of G, else rows y and batch z)
+        // (twin: the two segments of pair 0 are equal, and pair 2 equals pair 1)
         static const int ZPAIR[3][2] = {{0, 2}, {1, 4}, {3, 5}};
         for (int pr = 0; pr < 3 && he == hipSuccess; pr++) {
+            if (twin && pr == 2)
+                continue;
             const int64_t r = occ[PERM[ZPAIR[pr][0]][2]], sz = pos_stride[PERM[ZPAIR[pr][0]][2]];
             GemmArgs g{};
-            g.nseg = 2;
+            g.nseg = twin && pr == 0 ? 1 : 2;
             g.M = (int)v;
             g.N = (int)o;
-            g.alpha = 1.0;
+            g.alpha = twin ? 2.0 : 1.0;
             g.splits = 1;
-            for (int h = 0; h < 2; h++) {
+            for (int h = 0; h < g.nseg; h++) {
                 const int sgm = ZPAIR[pr][h];
                 const int64_t p = occ[PERM[sgm][0]], q = occ[PERM[sgm][1]];
                 const int64_t sx = pos_stride[PERM[sgm][0]], sy = pos_stride[PERM[sgm][1]];
