@@ -878,7 +878,8 @@ std::vector<double> matmul(const std::vector<double> &A, const std::vector<doubl
     return C;
 }
 
-// Solves the symmetric system M x = r (m x m) by Gaussian elimination with partial pivoting
+// Solves the symmetric system M x = r (m x m) by Gaussian elimination with partial pivoting; an empty
+// result when M is singular
 std::vector<double> solve(std::vector<double> M, std::vector<double> r, int m) {
     for (int c = 0; c < m; c++) {
         int piv = c;
@@ -890,7 +891,7 @@ std::vector<double> solve(std::vector<double> M, std::vector<double> r, int m) {
         std::swap(r[c], r[piv]);
         const double d = M[(size_t)c * m + c];
         if (std::fabs(d) < 1e-300)
-            throw std::runtime_error("RHF: singular DIIS system");
+            return {};
         for (int i = c + 1; i < m; i++) {
             const double f = M[(size_t)i * m + c] / d;
             for (int j = c; j < m; j++)
@@ -1030,11 +1031,14 @@ HF_Wfn RHF(const Molecule &molecule, const Basis::BasisSet &basis, double conver
                 for (int i = 0; i < m - 1; i++)
                     B[(size_t)i * m + m - 1] = B[(size_t)(m - 1) * m + i] = -1.0;
                 r[m - 1] = -1.0;
+                // a singular system (two equal error vectors, say) leaves this iteration's Fock matrix as it is
                 const std::vector<double> cdiis = solve(B, r, m);
-                std::fill(F.begin(), F.end(), 0.0);
-                for (int i = 0; i < m - 1; i++)
-                    for (size_t k = 0; k < F.size(); k++)
-                        F[k] += cdiis[i] * fs[i][k];
+                if (!cdiis.empty()) {
+                    std::fill(F.begin(), F.end(), 0.0);
+                    for (int i = 0; i < m - 1; i++)
+                        for (size_t k = 0; k < F.size(); k++)
+                            F[k] += cdiis[i] * fs[i][k];
+                }
             }
         }
         diagonalise(F, eps, C);
