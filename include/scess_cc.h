@@ -100,6 +100,40 @@ RCCSD_Wfn SolveRCCSD_DIIS(const TEITensor &eri, const std::vector<double> &orbit
 double RCCSD_TriplesCorrection(const TEITensor &eri, const std::vector<double> &orbitalEnergies, int64_t nOcc,
                                const T1Tensor &t1, const T2Tensor &t2, const Thread *thread = nullptr);
 
+// Gradients of E(T) (no counterpart in the reference): with respect to the three integral blocks the
+// functional reads (the repack of RCCSD_T.cpp:37-51: ovvv(i,a,b,f) = (ia|bf) = eri(i, o+b, o+a, o+f),
+// ovoo(i,a,m,j) = (ia|mj) = eri(i, m, o+a, j), ovov(i,a,j,b) = (ia|jb) = eri(i, j, o+a, o+b)), with
+// respect to <pq|rs> itself (those three index patterns are disjoint, so eri holds their gradients
+// scattered back and zeros elsewhere), and with respect to t1 and t2.
+struct TriplesGradient {
+    Tensor<4> ovvv, ovoo, ovov;
+    TEITensor eri;
+    T1Tensor t1;
+    T2Tensor t2;
+};
+
+// Autograd-Function-style forward/backward pair for (T) (no counterpart in the reference).
+// forward() is RCCSD_TriplesCorrection (same arguments, same E(T), same cancellation: 0.0) and keeps
+// the inputs resident on the GPU; backward(dE) returns dE times the gradients of that E(T), W being
+// recomputed per occupied triple in the precision given to the constructor (FP64 by default), the
+// back-contractions in FP64. backward() before a successful forward() throws; a cancelled
+// backward() returns an empty TriplesGradient. One object holds one GPU context; it is not copyable.
+class TriplesFunction {
+public:
+    enum class Precision { FP64, FP32, BF16X3 };
+    explicit TriplesFunction(Precision wRecompute = Precision::FP64, int device = 0);
+    ~TriplesFunction();
+    TriplesFunction(const TriplesFunction &) = delete;
+    TriplesFunction &operator=(const TriplesFunction &) = delete;
+    double forward(const TEITensor &eri, const std::vector<double> &orbitalEnergies, int64_t nOcc,
+                   const T1Tensor &t1, const T2Tensor &t2, const Thread *thread = nullptr);
+    TriplesGradient backward(double dE = 1.0, const Thread *thread = nullptr);
+
+private:
+    struct State;
+    State *s_;
+};
+
 // The CC part of RCCSD_T (src/Quantum/CC/RCCSD_T.cpp:170-195) from MO integrals and the HF energy:
 // CCSD, then (T) only when CCSD converged, with the reference's console output.
 RCCSD_Wfn RCCSD_T_FromMO(const TEITensor &eri, const std::vector<double> &orbitalEnergies, int64_t nOcc,

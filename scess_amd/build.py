@@ -15,6 +15,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libscess_gpu.so")
 CLI = os.path.join(HERE, "scess_cc")
+CPP_CHECK = os.path.join(ROOT, "tests", "cpp", "triples_function_check")
 SOURCES = ["triples_kernels.hip", "triples_adjoint.hip", "ccsd_engine.hip", "scess_capi.hip", "cc_api.cpp",
            "scf_host.cpp"]
 ARCH = "gfx950"
@@ -59,6 +60,10 @@ def build(verbose: bool = False) -> str:
     _run(["g++", "-O2", "-std=c++17", "-Wall", "-o", CLI + ".tmp", os.path.join(CSRC, "scess_cc_main.cpp"),
           "-L", HERE, "-lscess_gpu", "-Wl,-rpath,$ORIGIN"])
     os.replace(CLI + ".tmp", CLI)
+    # C++ check of the SCESS::CC method API (TriplesFunction), run on the GPU by tests/test_cpp_api_gpu.py
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), "-o", CPP_CHECK + ".tmp",
+          CPP_CHECK + ".cpp", "-L", HERE, "-lscess_gpu", "-Wl,-rpath,$ORIGIN/../../scess_amd"])
+    os.replace(CPP_CHECK + ".tmp", CPP_CHECK)
     return LIB
 
 

@@ -193,3 +193,98 @@ RCCSD_Wfn run_ccsd_t(const TEITensor &eri, const std::vector<double> &e, int64_t
 }
 
 }  // namespace SCESS::CC
+
+namespace SCESS::CC {
+
+struct TriplesFunction::State {
+    scess_ctx *c = nullptr;
+    int prec = SCESS_PREC_FP64;  // W recomputation of the backward
+    int64_t o = 0, v = 0, n = 0;
+    bool have = false;  // inputs of a successful forward are resident
+    void check(int rc) const {
+        if (rc != SCESS_OK && rc != SCESS_CANCELLED)
+            throw std::runtime_error(std::string("SCESS: ") + scess_last_error(c));
+    }
+};
+
+TriplesFunction::TriplesFunction(Precision wRecompute, int device) : s_(new State) {
+    s_->prec = wRecompute == Precision::FP32     ? SCESS_PREC_FP32
+               : wRecompute == Precision::BF16X3 ? SCESS_PREC_BF16X3
+                                                 : SCESS_PREC_FP64;
+    if (scess_create(&s_->c, device) != SCESS_OK) {
+        delete s_;
+        throw std::runtime_error("SCESS: no usable GPU (scess_create failed)");
+    }
+}
+
+TriplesFunction::~TriplesFunction() {
+    scess_destroy(s_->c);
+    delete s_;
+}
+
+double TriplesFunction::forward(const TEITensor &eri, const std::vector<double> &e, int64_t nOcc, const T1Tensor &t1,
+                                const T2Tensor &t2, const Thread *thread) {
+    const int64_t o = nOcc, n = eri.dimension(0), v = n - o;
+    s_->have = false;
+    if (eri.dimension(1) != n || eri.dimension(2) != n || eri.dimension(3) != n || (int64_t)e.size() < n || o < 0 ||
+        v < 0 || t1.dimension(0) != o || t1.dimension(1) != v || t2.dimension(0) != o || t2.dimension(1) != o ||
+        t2.dimension(2) != v || t2.dimension(3) != v)
+        throw std::runtime_error("SCESS: TriplesFunction::forward: inconsistent dimensions");
+    // the elements of the repack of RCCSD_T.cpp:37-51, as RCCSD_TriplesCorrection
+    const auto ovvv = block(o, v, v, v, [&](int64_t i, int64_t a, int64_t b, int64_t f) { return eri(i, o + b, o + a, o + f); });
+    const auto ovoo = block(o, v, o, o, [&](int64_t i, int64_t a, int64_t m, int64_t j) { return eri(i, m, o + a, j); });
+    const auto ovov = block(o, v, o, v, [&](int64_t i, int64_t a, int64_t j, int64_t b) { return eri(i, j, o + a, o + b); });
+    const scess_t_in in{o, v, e.data(), t1.data(), t2.data(), ovvv.data(), ovoo.data(), ovov.data()};
+    s_->check(scess_set_should_stop(s_->c, stop_cb, (void *)thread));
+    s_->check(scess_t_set_inputs(s_->c, &in, (1 << SCESS_PREC_FP64) | (1 << s_->prec)));
+    double E = 0.0;
+    const int rc = scess_t_forward(s_->c, SCESS_PREC_FP64, &E, nullptr);
+    (void)scess_set_should_stop(s_->c, nullptr, nullptr);  // the caller's thread may not outlive this call
+    if (rc == SCESS_CANCELLED)
+        return 0.0;
+    s_->check(rc);
+    s_->o = o;
+    s_->v = v;
+    s_->n = n;
+    s_->have = true;
+    return E;
+}
+
+TriplesGradient TriplesFunction::backward(double dE, const Thread *thread) {
+    if (!s_->have)
+        throw std::runtime_error("SCESS: TriplesFunction::backward needs a successful forward first");
+    const int64_t o = s_->o, v = s_->v, n = s_->n;
+    TriplesGradient g;
+    g.ovvv = Tensor<4>(o, v, v, v);
+    g.ovoo = Tensor<4>(o, v, o, o);
+    g.ovov = Tensor<4>(o, v, o, v);
+    g.t1 = T1Tensor(o, v);
+    g.t2 = T2Tensor(o, o, v, v);
+    scess_t_grad gr{g.ovvv.data(), g.ovoo.data(), g.ovov.data(), g.t1.data(), g.t2.data()};
+    s_->check(scess_set_should_stop(s_->c, stop_cb, (void *)thread));
+    const int rc = scess_t_backward(s_->c, s_->prec, dE, &gr);
+    (void)scess_set_should_stop(s_->c, nullptr, nullptr);
+    if (rc == SCESS_CANCELLED)
+        return {};
+    s_->check(rc);
+    // each block element reads one entry of <pq|rs>, and the three index patterns are disjoint
+    g.eri = TEITensor(n, n, n, n);
+    for (int64_t f = 0; f < v; f++)
+        for (int64_t b = 0; b < v; b++)
+            for (int64_t a = 0; a < v; a++)
+                for (int64_t i = 0; i < o; i++)
+                    g.eri(i, o + b, o + a, o + f) = g.ovvv(i, a, b, f);
+    for (int64_t j = 0; j < o; j++)
+        for (int64_t m = 0; m < o; m++)
+            for (int64_t a = 0; a < v; a++)
+                for (int64_t i = 0; i < o; i++)
+                    g.eri(i, m, o + a, j) = g.ovoo(i, a, m, j);
+    for (int64_t b = 0; b < v; b++)
+        for (int64_t j = 0; j < o; j++)
+            for (int64_t a = 0; a < v; a++)
+                for (int64_t i = 0; i < o; i++)
+                    g.eri(i, j, o + a, o + b) = g.ovov(i, a, j, b);
+    return g;
+}
+
+}  // namespace SCESS::CC

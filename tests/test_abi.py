@@ -47,5 +47,6 @@ def test_library_exports_the_cxx_method_api():
     for n in ("SCESS::CC::SolveRCCSD_DIIS(", "SCESS::CC::RCCSD_TriplesCorrection(", "SCESS::CC::RCCSD_T(",
               "SCESS::CC::RCCSD_T_FromMO(", "SCESS::Integrals::TransformTEI(", "SCESS::HF::RHF(",
               "SCESS::Integrals::EvaluateMolecularIntegrals(", "SCESS::Basis::LoadBasisSet(",
-              "SCESS::Molecule::GenerateWaterMolecule("):
+              "SCESS::Molecule::GenerateWaterMolecule(", "SCESS::CC::TriplesFunction::TriplesFunction(",
+              "SCESS::CC::TriplesFunction::forward(", "SCESS::CC::TriplesFunction::backward("):
         assert n in out, n
