@@ -221,6 +221,18 @@ RCCSD_Wfn RCCSD_T(const HF::HF_Wfn &rhfWfn, const Integrals::MolInts &mints, con
                   std::stringstream *console = nullptr);
 
 }  // namespace CC
+
+namespace Driver {
+// src/SCESS/Driver.h:13-40, Driver.cpp:51-65,173-231: RHF on the basis set (a BSE JSON name or path, as
+// Basis::LoadBasisSet), then the method, printing the reference's blocks to stdout ("--- Performing ...
+// calculation on <moleculeName> ---", the SCF and CC logs, the single-point energy summary). The SCF is
+// converged to 1e-10 (the reference's drivers use 1e-8) so that the CC energies carry no SCF residue.
+// The reference's drivers return void; these return whether the method converged (RHF: true when it
+// returns). Errors propagate as exceptions.
+bool RHF(const char *basisSetName, const Molecule &mol, const char *moleculeName = nullptr);
+bool CCSD(const char *basisSetName, const Molecule &mol, const char *moleculeName = nullptr);
+bool CCSD_T(const char *basisSetName, const Molecule &mol, const char *moleculeName = nullptr);
+}  // namespace Driver
 }  // namespace SCESS
 
 #endif /* SCESS_CC_H */

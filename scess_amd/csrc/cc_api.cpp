@@ -3,6 +3,7 @@
 #include "../../include/scess_gpu.h"
 
 #include <iomanip>
+#include <iostream>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -288,3 +289,59 @@ TriplesGradient TriplesFunction::backward(double dE, const Thread *thread) {
 }
 
 }  // namespace SCESS::CC
+
+namespace SCESS::Driver {
+namespace {
+
+enum class Method { RHF, CCSD, CCSD_T };
+
+bool run(Method method, const char *basisSetName, const Molecule &mol, const char *moleculeName) {
+    const char *title = moleculeName ? moleculeName : "unknown";
+    const char *what = method == Method::RHF ? "RHF" : method == Method::CCSD ? "CCSD" : "CCSD(T)";
+    std::cout << "--- Performing " << what << " calculation on " << title << " ---\n" << std::setprecision(12);
+    const Basis::BasisSet basis = Basis::LoadBasisSet(basisSetName ? basisSetName : "");
+    Integrals::MolInts mints;
+    std::stringstream console;
+    const HF::HF_Wfn hf = HF::RHF(mol, basis, 1e-10, 100, method == Method::RHF ? nullptr : &mints, nullptr, &console);
+    std::cout << console.str() << "\n";
+    console.str("");
+    if (method == Method::RHF) {
+        std::cout << "\n\n--- Restricted Hartree-Fock Single Point Energy ---\n  Molecule = " << title
+                  << "\n  E(RHF) = " << hf.E_HF << " hartrees\n";
+        return true;
+    }
+    CC::RCCSD_Wfn cc;
+    if (method == Method::CCSD_T) {
+        cc = CC::RCCSD_T(hf, mints, nullptr, &console);
+    } else {  // RCCSD_DIIS (CC.h:144-146): the MO transform, then SolveRCCSD_DIIS
+        int64_t nocc = 0;
+        while (nocc < (int64_t)hf.Occ_Alpha.size() && hf.Occ_Alpha[(size_t)nocc])
+            nocc++;
+        const CC::TEITensor eri = Integrals::TransformTEI(mints.eri, hf.C_Alpha, hf.nmo, true);
+        cc = CC::SolveRCCSD_DIIS(eri, hf.Ep_Alpha, nocc, nullptr, nullptr, &console);
+    }
+    std::cout << console.str() << "\n";
+    if (method == Method::CCSD_T)
+        std::cout << "\n\n--- CCSD(T) Single Point Energy ---\n  Molecule = " << title << "\n  E(RHF) = " << hf.E_HF
+                  << " hartrees\n  E(CCSD(T) correlation) = " << cc.E_CCSD + cc.E_T
+                  << " hartrees\n  E(Total) = " << cc.E_CCSD + hf.E_HF + cc.E_T << " hartrees\n";
+    else
+        std::cout << "\n\n--- Coupled-Cluster Singles & Doubles Single Point Energy ---\n  Molecule = " << title
+                  << "\n  E(RHF) = " << hf.E_HF << " hartrees\n  E(CCSD correlation) = " << cc.E_CCSD
+                  << " hartrees\n  E(Total) = " << cc.E_CCSD + hf.E_HF << " hartrees\n";
+    return cc.converged;
+}
+
+}  // namespace
+
+bool RHF(const char *basisSetName, const Molecule &mol, const char *moleculeName) {
+    return run(Method::RHF, basisSetName, mol, moleculeName);
+}
+bool CCSD(const char *basisSetName, const Molecule &mol, const char *moleculeName) {
+    return run(Method::CCSD, basisSetName, mol, moleculeName);
+}
+bool CCSD_T(const char *basisSetName, const Molecule &mol, const char *moleculeName) {
+    return run(Method::CCSD_T, basisSetName, mol, moleculeName);
+}
+
+}  // namespace SCESS::Driver

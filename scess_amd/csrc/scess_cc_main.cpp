@@ -82,45 +82,15 @@ Dump read_fcidump(const std::string &path) {
 
 // Driver.cpp:51-65 (rhf), 173-200 (ccsd), 203-231 (ccsd_t) with a built-in molecule and a basis set
 int run_molecule(const std::string &method, const std::string &molName, const std::string &basisName) {
+    // the names the reference's main passes to its drivers (Application.cpp:176-194)
     static const std::map<std::string, std::string> longName = {
         {"H2", "Hydrogen (H2)"}, {"LiH", "Lithium Hydride (LiH)"}, {"H2O", "Water (H2O)"}, {"NH3", "Ammonia (NH3)"}};
     const SCESS::Molecule mol = SCESS::Molecule::FromName(molName);
-    const SCESS::Basis::BasisSet basis = SCESS::Basis::LoadBasisSet(basisName);
     const std::string title = longName.at(molName);
-    const char *what = method == "rhf" ? "RHF" : method == "ccsd" ? "CCSD" : "CCSD(T)";
-    std::cout << "--- Performing " << what << " calculation on " << title << " ---\n";
-    std::cout << std::setprecision(12);
-    SCESS::Integrals::MolInts mints;
-    std::stringstream console;
-    // tighter than the reference driver's 1e-8, so that the CC energies carry no SCF residue
-    const SCESS::HF::HF_Wfn hf = SCESS::HF::RHF(mol, basis, 1e-10, 100, &mints, nullptr, &console);
-    std::cout << console.str() << "\n";
-    console.str("");
-    if (method == "rhf") {
-        std::cout << "\n\n--- Restricted Hartree-Fock Single Point Energy ---\n  Molecule = " << title
-                  << "\n  E(RHF) = " << hf.E_HF << " hartrees\n";
-        return 0;
-    }
-    SCESS::CC::RCCSD_Wfn cc;
-    if (method == "ccsd_t") {
-        cc = SCESS::CC::RCCSD_T(hf, mints, nullptr, &console);
-    } else {
-        int64_t nocc = 0;
-        while (nocc < (int64_t)hf.Occ_Alpha.size() && hf.Occ_Alpha[(size_t)nocc])
-            nocc++;
-        const SCESS::CC::TEITensor eri = SCESS::Integrals::TransformTEI(mints.eri, hf.C_Alpha, hf.nmo, true);
-        cc = SCESS::CC::SolveRCCSD_DIIS(eri, hf.Ep_Alpha, nocc, nullptr, nullptr, &console);
-    }
-    std::cout << console.str() << "\n";
-    if (method == "ccsd_t")
-        std::cout << "\n\n--- CCSD(T) Single Point Energy ---\n  Molecule = " << title << "\n  E(RHF) = " << hf.E_HF
-                  << " hartrees\n  E(CCSD(T) correlation) = " << cc.E_CCSD + cc.E_T
-                  << " hartrees\n  E(Total) = " << cc.E_CCSD + hf.E_HF + cc.E_T << " hartrees\n";
-    else
-        std::cout << "\n\n--- Coupled-Cluster Singles & Doubles Single Point Energy ---\n  Molecule = " << title
-                  << "\n  E(RHF) = " << hf.E_HF << " hartrees\n  E(CCSD correlation) = " << cc.E_CCSD
-                  << " hartrees\n  E(Total) = " << cc.E_CCSD + hf.E_HF << " hartrees\n";
-    return cc.converged ? 0 : 1;
+    const bool ok = method == "rhf"    ? SCESS::Driver::RHF(basisName.c_str(), mol, title.c_str())
+                    : method == "ccsd" ? SCESS::Driver::CCSD(basisName.c_str(), mol, title.c_str())
+                                       : SCESS::Driver::CCSD_T(basisName.c_str(), mol, title.c_str());
+    return ok ? 0 : 1;
 }
 
 }  // namespace

@@ -48,5 +48,6 @@ def test_library_exports_the_cxx_method_api():
               "SCESS::CC::RCCSD_T_FromMO(", "SCESS::Integrals::TransformTEI(", "SCESS::HF::RHF(",
               "SCESS::Integrals::EvaluateMolecularIntegrals(", "SCESS::Basis::LoadBasisSet(",
               "SCESS::Molecule::GenerateWaterMolecule(", "SCESS::CC::TriplesFunction::TriplesFunction(",
-              "SCESS::CC::TriplesFunction::forward(", "SCESS::CC::TriplesFunction::backward("):
+              "SCESS::CC::TriplesFunction::forward(", "SCESS::CC::TriplesFunction::backward(",
+              "SCESS::Driver::RHF(", "SCESS::Driver::CCSD(", "SCESS::Driver::CCSD_T("):
         assert n in out, n
