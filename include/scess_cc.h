@@ -220,6 +220,11 @@ namespace CC {
 RCCSD_Wfn RCCSD_T(const HF::HF_Wfn &rhfWfn, const Integrals::MolInts &mints, const Thread *thread = nullptr,
                   std::stringstream *console = nullptr);
 
+// RCCSD_DIIS (src/Quantum/CC/CC.h:143-146, RCCSD.cpp:396-405): nOcc from Occ_Alpha, <pq|rs> =
+// TransformTEI(mints.eri, C_Alpha, true), SolveRCCSD_DIIS, and the total-energy line when converged.
+RCCSD_Wfn RCCSD_DIIS(const HF::HF_Wfn &rhfWfn, const Integrals::MolInts &mints, const Thread *thread = nullptr,
+                     std::stringstream *console = nullptr);
+
 }  // namespace CC
 
 namespace Driver {

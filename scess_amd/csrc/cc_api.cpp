@@ -151,6 +151,19 @@ RCCSD_Wfn RCCSD_T(const HF::HF_Wfn &rhfWfn, const Integrals::MolInts &mints, con
     return run_ccsd_t(eri, rhfWfn.Ep_Alpha, nOcc, rhfWfn.E_HF, thread, console);
 }
 
+RCCSD_Wfn RCCSD_DIIS(const HF::HF_Wfn &rhfWfn, const Integrals::MolInts &mints, const Thread *thread,
+                     std::stringstream *console) {
+    int64_t nOcc = 0;  // position of the first unoccupied orbital (RCCSD.cpp:398)
+    while (nOcc < (int64_t)rhfWfn.Occ_Alpha.size() && rhfWfn.Occ_Alpha[nOcc] != 0)
+        nOcc++;
+    const TEITensor eri = Integrals::TransformTEI(mints.eri, rhfWfn.C_Alpha, rhfWfn.nmo, true);
+    RCCSD_Wfn wfn = SolveRCCSD_DIIS(eri, rhfWfn.Ep_Alpha, nOcc, nullptr, thread, console);
+    if (console && wfn.converged)
+        *console << std::setprecision(12) << "E(Total) = E(Hartree Fock) + E(CCSD) = " << rhfWfn.E_HF + wfn.E_CCSD
+                 << " hartrees\n\n\n";
+    return wfn;
+}
+
 }  // namespace SCESS::CC
 
 namespace SCESS::Integrals {
@@ -313,12 +326,8 @@ bool run(Method method, const char *basisSetName, const Molecule &mol, const cha
     CC::RCCSD_Wfn cc;
     if (method == Method::CCSD_T) {
         cc = CC::RCCSD_T(hf, mints, nullptr, &console);
-    } else {  // RCCSD_DIIS (CC.h:144-146): the MO transform, then SolveRCCSD_DIIS
-        int64_t nocc = 0;
-        while (nocc < (int64_t)hf.Occ_Alpha.size() && hf.Occ_Alpha[(size_t)nocc])
-            nocc++;
-        const CC::TEITensor eri = Integrals::TransformTEI(mints.eri, hf.C_Alpha, hf.nmo, true);
-        cc = CC::SolveRCCSD_DIIS(eri, hf.Ep_Alpha, nocc, nullptr, nullptr, &console);
+    } else {
+        cc = CC::RCCSD_DIIS(hf, mints, nullptr, &console);
     }
     std::cout << console.str() << "\n";
     if (method == Method::CCSD_T)

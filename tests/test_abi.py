@@ -49,5 +49,5 @@ def test_library_exports_the_cxx_method_api():
               "SCESS::Integrals::EvaluateMolecularIntegrals(", "SCESS::Basis::LoadBasisSet(",
               "SCESS::Molecule::GenerateWaterMolecule(", "SCESS::CC::TriplesFunction::TriplesFunction(",
               "SCESS::CC::TriplesFunction::forward(", "SCESS::CC::TriplesFunction::backward(",
-              "SCESS::Driver::RHF(", "SCESS::Driver::CCSD(", "SCESS::Driver::CCSD_T("):
+              "SCESS::Driver::RHF(", "SCESS::Driver::CCSD(", "SCESS::Driver::CCSD_T(", "SCESS::CC::RCCSD_DIIS("):
         assert n in out, n
