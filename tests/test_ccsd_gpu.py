@@ -207,6 +207,30 @@ def test_cli_builtin_water(tmp_path, method, basis):
     assert abs(total - (ehf + corr)) < 1e-9
 
 
+def test_cli_builtin_hydrogen(tmp_path):
+    """`scess_cc ccsd_t H2 STO-3G`: one occupied and one virtual orbital. CCSD is exact for two electrons,
+    so E(RHF) + E(CCSD) is the FCI energy (Tests.cpp:34-38), and (T) vanishes with one occupied orbital."""
+    import os
+    import re
+    import subprocess
+    from test_cli import make_basis_dir
+    _say("case cli builtin H2 ccsd_t STO-3G (o=1, v=1)")
+    cli = os.path.join(os.path.dirname(gpu.LIB_PATH), "scess_cc")
+    env = dict(os.environ, SCESS_BASIS_DIR=make_basis_dir(tmp_path))
+    r = subprocess.run([cli, "ccsd_t", "H2", "STO-3G"], capture_output=True, text=True, timeout=120, env=env)
+    _say(r.stdout[-700:], r.stderr[-600:])
+    assert r.returncode == 0
+    num = r"(-?[0-9.]+(?:e[-+]?\d+)?)"
+    ehf = float(re.search(r"E\(RHF\) = " + num, r.stdout).group(1))
+    et = float(re.search(r"E\(T\) = " + num, r.stdout).group(1))
+    corr = float(re.search(r"E\(CCSD\(T\) correlation\) = " + num, r.stdout).group(1))
+    total = float(re.search(r"E\(Total\) = " + num + " hartrees\n*$", r.stdout).group(1))
+    assert abs(ehf - (-1.1175058851)) < 1e-8
+    assert et == 0.0
+    assert abs(corr - (-0.0193419373)) < 1e-8
+    assert abs(total - (-1.1368478224)) < 1e-8
+
+
 def test_water_baseline_geometry_gpu_vs_oracle(ctx):
     """Water at O-H 0.9572 A, H-O-H 104.52 deg, cc-pVDZ: CCSD and (T) on the GPU against the CPU
     oracles (1e-10 Eh); no published energy exists for this geometry."""
