@@ -113,20 +113,23 @@ struct TriplesGradient {
 };
 
 // Autograd-Function-style forward/backward pair for (T) (no counterpart in the reference).
-// forward() is RCCSD_TriplesCorrection (same arguments, same E(T), same cancellation: 0.0) and keeps
-// the inputs resident on the GPU; backward(dE) returns dE times the gradients of that E(T), W being
-// recomputed per occupied triple in the precision given to the constructor (FP64 by default), the
-// back-contractions in FP64. backward() before a successful forward() throws; a cancelled
-// backward() returns an empty TriplesGradient. One object holds one GPU context; it is not copyable.
+// forward() is RCCSD_TriplesCorrection (same arguments, same E(T) in FP64, same cancellation: 0.0)
+// and keeps the inputs resident on the GPU; its contractions can also run in FP32, split-BF16x3 or
+// exact int8-digit products (OZAKI), with FP64 denominators and sums. backward(dE) returns dE times
+// the gradients of E(T), W being recomputed per occupied triple in the precision given to the
+// constructor (FP64, FP32 or BF16X3; FP64 by default), the back-contractions in FP64. backward()
+// before a successful forward() throws; a cancelled backward() returns an empty TriplesGradient.
+// One object holds one GPU context; it is not copyable.
 class TriplesFunction {
 public:
-    enum class Precision { FP64, FP32, BF16X3 };
+    enum class Precision { FP64, FP32, BF16X3, OZAKI };
     explicit TriplesFunction(Precision wRecompute = Precision::FP64, int device = 0);
     ~TriplesFunction();
     TriplesFunction(const TriplesFunction &) = delete;
     TriplesFunction &operator=(const TriplesFunction &) = delete;
     double forward(const TEITensor &eri, const std::vector<double> &orbitalEnergies, int64_t nOcc,
-                   const T1Tensor &t1, const T2Tensor &t2, const Thread *thread = nullptr);
+                   const T1Tensor &t1, const T2Tensor &t2, const Thread *thread = nullptr,
+                   Precision precision = Precision::FP64);
     TriplesGradient backward(double dE = 1.0, const Thread *thread = nullptr);
 
 private:

@@ -221,10 +221,23 @@ struct TriplesFunction::State {
     }
 };
 
+namespace {
+int prec_code(TriplesFunction::Precision p) {
+    switch (p) {
+    case TriplesFunction::Precision::FP32: return SCESS_PREC_FP32;
+    case TriplesFunction::Precision::BF16X3: return SCESS_PREC_BF16X3;
+    case TriplesFunction::Precision::OZAKI: return SCESS_PREC_OZAKI;
+    default: return SCESS_PREC_FP64;
+    }
+}
+}  // namespace
+
 TriplesFunction::TriplesFunction(Precision wRecompute, int device) : s_(new State) {
-    s_->prec = wRecompute == Precision::FP32     ? SCESS_PREC_FP32
-               : wRecompute == Precision::BF16X3 ? SCESS_PREC_BF16X3
-                                                 : SCESS_PREC_FP64;
+    if (wRecompute == Precision::OZAKI) {
+        delete s_;
+        throw std::runtime_error("SCESS: TriplesFunction: the W recomputation of the backward takes FP64, FP32 or BF16X3");
+    }
+    s_->prec = prec_code(wRecompute);
     if (scess_create(&s_->c, device) != SCESS_OK) {
         delete s_;
         throw std::runtime_error("SCESS: no usable GPU (scess_create failed)");
@@ -237,7 +250,8 @@ TriplesFunction::~TriplesFunction() {
 }
 
 double TriplesFunction::forward(const TEITensor &eri, const std::vector<double> &e, int64_t nOcc, const T1Tensor &t1,
-                                const T2Tensor &t2, const Thread *thread) {
+                                const T2Tensor &t2, const Thread *thread, Precision precision) {
+    const int fprec = prec_code(precision);
     const int64_t o = nOcc, n = eri.dimension(0), v = n - o;
     s_->have = false;
     if (eri.dimension(1) != n || eri.dimension(2) != n || eri.dimension(3) != n || (int64_t)e.size() < n || o < 0 ||
@@ -250,9 +264,9 @@ double TriplesFunction::forward(const TEITensor &eri, const std::vector<double> 
     const auto ovov = block(o, v, o, v, [&](int64_t i, int64_t a, int64_t j, int64_t b) { return eri(i, j, o + a, o + b); });
     const scess_t_in in{o, v, e.data(), t1.data(), t2.data(), ovvv.data(), ovoo.data(), ovov.data()};
     s_->check(scess_set_should_stop(s_->c, stop_cb, (void *)thread));
-    s_->check(scess_t_set_inputs(s_->c, &in, (1 << SCESS_PREC_FP64) | (1 << s_->prec)));
+    s_->check(scess_t_set_inputs(s_->c, &in, (1 << SCESS_PREC_FP64) | (1 << s_->prec) | (1 << fprec)));
     double E = 0.0;
-    const int rc = scess_t_forward(s_->c, SCESS_PREC_FP64, &E, nullptr);
+    const int rc = scess_t_forward(s_->c, fprec, &E, nullptr);
     (void)scess_set_should_stop(s_->c, nullptr, nullptr);  // the caller's thread may not outlive this call
     if (rc == SCESS_CANCELLED)
         return 0.0;

@@ -4,7 +4,8 @@
 // dE/d<pq|rs>, dE/dt1 and dE/dt2 agree with central differences of RCCSD_TriplesCorrection (E is
 // quadratic in each entry, so the difference is exact up to rounding); dE/d<pq|rs> is zero outside the
 // three index patterns the functional reads; cancellation returns 0.0 and leaves no state to
-// differentiate. Prints one line per check; exit status 0 when all pass.
+// differentiate; the FP32, BF16X3 and OZAKI forwards stay within 1e-5, 1e-4 and 1e-12 (relative) of
+// FP64. Prints one line per check; exit status 0 when all pass.
 #include "scess_cc.h"
 
 #include <array>
@@ -157,6 +158,25 @@ int main() {
                                std::array<int64_t, 4>{1, 0, 4, 4}}) {
             const double an = g1.t2(ix[0], ix[1], ix[2], ix[3]), fd = fd_t2(ix[0], ix[1], ix[2], ix[3]);
             check(near(an, fd), "dE/dt2 vs central difference", an, fd);
+        }
+
+        // the forward in the other product precisions, against the FP64 E(T)
+        {
+            TriplesFunction tm;
+            using P = TriplesFunction::Precision;
+            const double Ef = tm.forward(eri, e, o, t1, t2, nullptr, P::FP32);
+            check(std::fabs(Ef - E) <= 1e-5 * std::fabs(E), "FP32 forward within 1e-5 (relative) of FP64", Ef, E);
+            const double Eb = tm.forward(eri, e, o, t1, t2, nullptr, P::BF16X3);
+            check(std::fabs(Eb - E) <= 1e-4 * std::fabs(E), "BF16X3 forward within 1e-4 (relative) of FP64", Eb, E);
+            const double Ez = tm.forward(eri, e, o, t1, t2, nullptr, P::OZAKI);
+            check(std::fabs(Ez - E) <= 1e-12 * std::fabs(E), "OZAKI forward within 1e-12 (relative) of FP64", Ez, E);
+            bool rejected = false;
+            try {
+                TriplesFunction bad(P::OZAKI);
+            } catch (const std::runtime_error &) {
+                rejected = true;
+            }
+            check(rejected, "OZAKI W recomputation rejected", rejected ? 1.0 : 0.0, 1.0);
         }
 
         // cancellation: forward returns 0.0 and leaves nothing to differentiate
