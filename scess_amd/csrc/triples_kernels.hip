@@ -1056,70 +1056,10 @@ struct OrbitRow {
     int step[5];    // Yl offset increment per c'
 };
 
-template <int PB, int PX, int PY>
-__device__ inline OrbitRow orbit_row(const int *orbits, int orb, int tid) {
-    constexpr int EB = 8;
-    OrbitRow r;
-    const int A = orbits[3 * orb], B = orbits[3 * orb + 1], C = orbits[3 * orb + 2];
-    const int bl1 = B != A ? B : C;
-    auto slot = [&](int blk) { return blk == A ? 0 : (blk == bl1 ? 1 : 2); };
-    const int sB = slot(B), sC = slot(C);
-    uint64_t map_lo = 0, map_hi = 0;
-    uint32_t seen = 0;
-    int nd = 0;
-    const int d = tid >> 6, ap = (tid >> 3) & 7, bp = tid & 7;
-    int P = 0, Q = 0, R = 0, sP = 0, sQ = 0, sR = 0;
-    auto add = [&](int p0, int p1, int p2, int s0, int s1, int s2) {
-        const int key = s0 * 9 + s1 * 3 + s2;
-        if (!(seen >> key & 1u)) {
-            seen |= 1u << key;
-            if (key < 21)
-                map_lo |= (uint64_t)nd << (3 * key);
-            else
-                map_hi |= (uint64_t)nd << (3 * (key - 21));
-            if (nd == d) {
-                P = p0;
-                Q = p1;
-                R = p2;
-                sP = s0;
-                sQ = s1;
-                sR = s2;
-            }
-            nd++;
-        }
-    };
-    add(A, B, C, 0, sB, sC);
-    add(B, C, A, sB, sC, 0);
-    add(C, A, B, sC, 0, sB);
-    add(C, B, A, sC, sB, 0);
-    add(A, C, B, 0, sC, sB);
-    add(B, A, C, sB, 0, sC);
-    auto dindex = [&](int s0, int s1, int s2) -> int {
-        const int key = s0 * 9 + s1 * 3 + s2;
-        return (int)((key < 21 ? map_lo >> (3 * key) : map_hi >> (3 * (key - 21))) & 7u);
-    };
-    r.nd = nd;
-    r.own = d < nd;
-    r.ga = P * EB + ap;
-    r.gb = Q * EB + bp;
-    r.gc0 = R * EB;
-    // Y(lb, lc, la), Y(lc, la, lb), Y(lc, lb, la), Y(la, lc, lb), Y(lb, la, lc) with lc = (sR, c')
-    r.off[0] = dindex(sQ, sR, sP) * PB + bp * PX + ap;
-    r.step[0] = PY;
-    r.off[1] = dindex(sR, sP, sQ) * PB + ap * PY + bp;
-    r.step[1] = PX;
-    r.off[2] = dindex(sR, sQ, sP) * PB + bp * PY + ap;
-    r.step[2] = PX;
-    r.off[3] = dindex(sP, sR, sQ) * PB + ap * PX + bp;
-    r.step[3] = PY;
-    r.off[4] = dindex(sQ, sP, sR) * PB + bp * PX + ap * PY;
-    r.step[4] = 1;
-    return r;
-}
-
-// The same geometry from the host-built table (one 64-bit entry per orbit and distinct block d):
+// The geometry from the host-built table (orbit_table: one 64-bit entry per orbit and distinct block d):
 // P, Q, R in bits 0-23, the number of distinct blocks in 24-26, and the distinct-block indices of
-// the five permuted reads in 27-41 (3 bits each).
+// the five permuted reads of r3 in 27-41 (3 bits each): Y(lb, lc, la), Y(lc, la, lb), Y(lc, lb, la),
+// Y(la, lc, lb), Y(lb, la, lc) with lc = (sR, c').
 template <int PB, int PX, int PY>
 __device__ inline OrbitRow orbit_row_tab(const uint64_t *tab, int orb, int tid) {
     constexpr int EB = 8;
