@@ -50,6 +50,9 @@ def _bind(lib):
     lib.oracle_t_grad.restype = None
     lib.oracle_t_grad.argtypes = [ctypes.c_int64, ctypes.c_int64, _dp, _dp, _dp, _dp, _dp, _dp,
                                   _dp, _dp, _dp, _dp, _dp]
+    lib.oracle_t_grad_items.restype = None
+    lib.oracle_t_grad_items.argtypes = [ctypes.c_int64, ctypes.c_int64, _dp, _dp, _dp, _dp, _dp, _dp,
+                                        _i64p, ctypes.c_int64, _dp, _dp, _dp, _dp, _dp]
     lib.oracle_num_unique_triples.restype = ctypes.c_int64
     lib.oracle_num_unique_triples.argtypes = [ctypes.c_int64]
     lib.oracle_num_unique_ijk.restype = ctypes.c_int64
@@ -152,13 +155,18 @@ def t_brute(inp, lib=None):
     return lib.oracle_t_brute(inp.o, inp.v, *[_p(a) for a in arrs])
 
 
-def t_grad(inp, lib=None):
-    """dict of FP64 gradients of E(T) in the input layouts."""
+def t_grad(inp, items=None, lib=None):
+    """dict of FP64 gradients of E(T) in the input layouts; with items (indices into unique_ijk's
+    order), the gradient of sum over those occupied triples of 2/deg F_ijk only."""
     lib = lib or load()
     arrs = _args(inp)
     g = {k: np.zeros(getattr(inp, k).shape, order="F") for k in ("ovvv", "ovoo", "ovov", "t1", "t2")}
-    lib.oracle_t_grad(inp.o, inp.v, *[_p(a) for a in arrs],
-                      _p(g["ovvv"]), _p(g["ovoo"]), _p(g["ovov"]), _p(g["t1"]), _p(g["t2"]))
+    outs = (_p(g["ovvv"]), _p(g["ovoo"]), _p(g["ovov"]), _p(g["t1"]), _p(g["t2"]))
+    if items is None:
+        lib.oracle_t_grad(inp.o, inp.v, *[_p(a) for a in arrs], *outs)
+    else:
+        it = np.ascontiguousarray(items, dtype=np.int64)
+        lib.oracle_t_grad_items(inp.o, inp.v, *[_p(a) for a in arrs], it.ctypes.data_as(_i64p), len(it), *outs)
     return g
 
 

@@ -590,11 +590,16 @@ double oracle_t_brute(int64_t o, int64_t v, const double *e, const double *t1, c
  * With r3 symmetric: dE/dW = w (r3(Y)/D + r3(W/D)) and dE/dV = w r3(W/D),
  * w = 2/deg_ijk, then through the six permutation terms to the array entries
  * each of them reads. Gradients are accumulated (+=) into the caller's arrays,
- * which use the input layouts. Single-threaded; meant for small sizes.
+ * which use the input layouts. items (nitems entries) selects unique occupied
+ * triples by their index in oracle_unique_ijk's order (NULL: all of them), so
+ * the result is the gradient of the F-weighted sum over those triples only.
+ * The f-sums run on the f-contiguous copies Xr, Tr with f-contiguous gradient
+ * buffers dXr, dTr, added into d_ovvv and d_t2 at the end. Single-threaded.
  */
-void oracle_t_grad(int64_t o, int64_t v, const double *e, const double *t1, const double *t2, const double *ovvv,
-                   const double *ovoo, const double *ovov, double *d_ovvv, double *d_ovoo, double *d_ovov,
-                   double *d_t1, double *d_t2) {
+void oracle_t_grad_items(int64_t o, int64_t v, const double *e, const double *t1, const double *t2,
+                         const double *ovvv, const double *ovoo, const double *ovov, const int64_t *items,
+                         int64_t nitems, double *d_ovvv, double *d_ovoo, double *d_ovov, double *d_t1,
+                         double *d_t2) {
     ijk_ctx_t C;
     ijk_ctx_init(&C, o, v, e, t1, t2, ovvv, ovoo, ovov);
     const idx_t v3 = v * v * v;
@@ -604,59 +609,85 @@ void oracle_t_grad(int64_t o, int64_t v, const double *e, const double *t1, cons
     double *WD = (double *)malloc(sizeof(double) * (size_t)v3);
     double *GW = (double *)malloc(sizeof(double) * (size_t)v3);
     double *GV = (double *)malloc(sizeof(double) * (size_t)v3);
-    for (idx_t i = 0; i < o; i++)
-        for (idx_t j = 0; j <= i; j++)
-            for (idx_t k = 0; k <= j; k++) {
-                if (i == k)
-                    continue;
-                const double wgt = (i == j || j == k) ? 1.0 : 2.0;
-                ijk_build_wv(&C, i, j, k, W, V);
-                const double eocc = e[i] + e[j] + e[k];
-                for (idx_t a = 0; a < v; a++)
-                    for (idx_t b = 0; b < v; b++)
-                        for (idx_t c = 0; c < v; c++) {
-                            const idx_t n = (a * v + b) * v + c;
-                            const double D = eocc - e[o + a] - e[o + b] - e[o + c];
-                            Y[n] = W[n] + V[n];
-                            WD[n] = W[n] / D;
-                        }
-                for (idx_t a = 0; a < v; a++)
-                    for (idx_t b = 0; b < v; b++)
-                        for (idx_t c = 0; c < v; c++) {
-                            const idx_t n = (a * v + b) * v + c;
-                            const double D = eocc - e[o + a] - e[o + b] - e[o + c];
-                            const double rw = r3v(WD, v, a, b, c);
-                            GW[n] = wgt * (r3v(Y, v, a, b, c) / D + rw);
-                            GV[n] = wgt * rw;
-                        }
-                const idx_t occ[3] = {i, j, k};
-                for (int s = 0; s < 6; s++) {
-                    const idx_t p = occ[PERM[s][0]], q = occ[PERM[s][1]], r = occ[PERM[s][2]];
-                    for (idx_t a = 0; a < v; a++)
-                        for (idx_t b = 0; b < v; b++)
-                            for (idx_t c = 0; c < v; c++) {
-                                const idx_t abc[3] = {a, b, c};
-                                const idx_t x = abc[PERM[s][0]], y = abc[PERM[s][1]], z = abc[PERM[s][2]];
-                                const double g = GW[(a * v + b) * v + c];
-                                for (idx_t f = 0; f < v; f++) {
-                                    const idx_t iX = p + o * (x + v * (y + v * f));
-                                    const idx_t iT = r + o * (q + o * (z + v * f));
-                                    d_ovvv[iX] += g * t2[iT];
-                                    d_t2[iT] += g * ovvv[iX];
-                                }
-                                for (idx_t m = 0; m < o; m++) {
-                                    const idx_t iR = p + o * (x + v * (m + o * q));
-                                    const idx_t iT = m + o * (r + o * (y + v * z));
-                                    d_ovoo[iR] -= g * t2[iT];
-                                    d_t2[iT] -= g * ovoo[iR];
-                                }
-                                const double gv = 0.5 * GV[(a * v + b) * v + c];
-                                const idx_t iS = p + o * (x + v * (q + o * y));
-                                d_ovov[iS] += gv * t1[r + o * z];
-                                d_t1[r + o * z] += gv * ovov[iS];
-                            }
+    double *dXr = (double *)calloc((size_t)(o * v3), sizeof(double));    /* [p][x][y][f] */
+    double *dTr = (double *)calloc((size_t)(o * o * v * v), sizeof(double)); /* [q][r][z][f] */
+    const idx_t nijk = oracle_num_unique_ijk(o);
+    int32_t *ijk = (int32_t *)malloc(sizeof(int32_t) * 3 * (size_t)(nijk > 0 ? nijk : 1));
+    oracle_unique_ijk(o, ijk);
+    const idx_t nit = items ? nitems : nijk;
+    for (idx_t it = 0; it < nit; it++) {
+        const int32_t *qq = ijk + 3 * (items ? items[it] : it);
+        const idx_t i = qq[0], j = qq[1], k = qq[2];
+        if (i == k)
+            continue;
+        const double wgt = (i == j || j == k) ? 1.0 : 2.0;
+        ijk_build_wv(&C, i, j, k, W, V);
+        const double eocc = e[i] + e[j] + e[k];
+        for (idx_t a = 0; a < v; a++)
+            for (idx_t b = 0; b < v; b++)
+                for (idx_t c = 0; c < v; c++) {
+                    const idx_t n = (a * v + b) * v + c;
+                    const double D = eocc - e[o + a] - e[o + b] - e[o + c];
+                    Y[n] = W[n] + V[n];
+                    WD[n] = W[n] / D;
                 }
-            }
+        for (idx_t a = 0; a < v; a++)
+            for (idx_t b = 0; b < v; b++)
+                for (idx_t c = 0; c < v; c++) {
+                    const idx_t n = (a * v + b) * v + c;
+                    const double D = eocc - e[o + a] - e[o + b] - e[o + c];
+                    const double rw = r3v(WD, v, a, b, c);
+                    GW[n] = wgt * (r3v(Y, v, a, b, c) / D + rw);
+                    GV[n] = wgt * rw;
+                }
+        const idx_t occ[3] = {i, j, k};
+        for (int s = 0; s < 6; s++) {
+            const idx_t p = occ[PERM[s][0]], q = occ[PERM[s][1]], r = occ[PERM[s][2]];
+            /* term s reads (x, y, z) = (abc[PERM[s][0]], abc[PERM[s][1]], abc[PERM[s][2]]) */
+            for (idx_t x = 0; x < v; x++)
+                for (idx_t y = 0; y < v; y++) {
+                    const double *X = C.Xr + ((p * v + x) * v + y) * v;
+                    double *dX = dXr + ((p * v + x) * v + y) * v;
+                    for (idx_t z = 0; z < v; z++) {
+                        idx_t abc[3];
+                        abc[PERM[s][0]] = x;
+                        abc[PERM[s][1]] = y;
+                        abc[PERM[s][2]] = z;
+                        const idx_t n = (abc[0] * v + abc[1]) * v + abc[2];
+                        const double g = GW[n];
+                        const double *T = C.Tr + ((r * o + q) * v + z) * v;
+                        double *dT = dTr + ((r * o + q) * v + z) * v;
+                        for (idx_t f = 0; f < v; f++) {
+                            dX[f] += g * T[f];
+                            dT[f] += g * X[f];
+                        }
+                        for (idx_t m = 0; m < o; m++) {
+                            const idx_t iR = p + o * (x + v * (m + o * q));
+                            const idx_t iT = m + o * (r + o * (y + v * z));
+                            d_ovoo[iR] -= g * t2[iT];
+                            d_t2[iT] -= g * ovoo[iR];
+                        }
+                        const double gv = 0.5 * GV[n];
+                        const idx_t iS = p + o * (x + v * (q + o * y));
+                        d_ovov[iS] += gv * t1[r + o * z];
+                        d_t1[r + o * z] += gv * ovov[iS];
+                    }
+                }
+        }
+    }
+    for (idx_t p = 0; p < o; p++)
+        for (idx_t x = 0; x < v; x++)
+            for (idx_t y = 0; y < v; y++)
+                for (idx_t f = 0; f < v; f++)
+                    d_ovvv[p + o * (x + v * (y + v * f))] += dXr[((p * v + x) * v + y) * v + f];
+    for (idx_t q = 0; q < o; q++)
+        for (idx_t r = 0; r < o; r++)
+            for (idx_t z = 0; z < v; z++)
+                for (idx_t f = 0; f < v; f++)
+                    d_t2[q + o * (r + o * (z + v * f))] += dTr[((q * o + r) * v + z) * v + f];
+    free(ijk);
+    free(dXr);
+    free(dTr);
     free(W);
     free(V);
     free(Y);
@@ -664,4 +695,10 @@ void oracle_t_grad(int64_t o, int64_t v, const double *e, const double *t1, cons
     free(GW);
     free(GV);
     ijk_ctx_free(&C);
+}
+
+void oracle_t_grad(int64_t o, int64_t v, const double *e, const double *t1, const double *t2, const double *ovvv,
+                   const double *ovoo, const double *ovov, double *d_ovvv, double *d_ovoo, double *d_ovov,
+                   double *d_t1, double *d_t2) {
+    oracle_t_grad_items(o, v, e, t1, t2, ovvv, ovoo, ovov, NULL, 0, d_ovvv, d_ovoo, d_ovov, d_t1, d_t2);
 }

@@ -387,7 +387,10 @@ int scess_t_set_inputs(scess_ctx *c, const scess_t_in *in, int prec_mask) {
     t.ngrp = epilogue_groups(t.norb);
     {
         std::vector<uint64_t> tab;
-        orbit_table(t.orbits, tab);
+        if (!orbit_table(t.orbits, tab)) {  // unreachable for v <= 4096 (nb <= 512)
+            free_triples(t);
+            return fail(c, SCESS_ERR_ARG, "scess_t_set_inputs: too many virtual blocks for the orbit table");
+        }
         HIP_TRY(c, hipMalloc((void **)&t.d_orbtab, sizeof(uint64_t) * tab.size()));
         HIP_TRY(c, hipMemcpy(t.d_orbtab, tab.data(), sizeof(uint64_t) * tab.size(), hipMemcpyHostToDevice));
     }

@@ -68,14 +68,18 @@ Dump read_fcidump(const std::string &path) {
             throw std::runtime_error("FCIDUMP index out of range");
         if (i == 0 && j == 0 && k == 0 && l == 0) {
             d.core = x;
-        } else if (k == 0 && l == 0) {
+        } else if (i > 0 && j > 0 && k == 0 && l == 0) {
             d.h[(i - 1) * n + (j - 1)] = d.h[(j - 1) * n + (i - 1)] = x;
         } else if (i > 0 && j > 0 && k > 0 && l > 0) {
             const int64_t p = i - 1, q = j - 1, r = k - 1, s = l - 1;
             E(p, q, r, s) = E(q, p, r, s) = E(p, q, s, r) = E(q, p, s, r) = x;
             E(r, s, p, q) = E(s, r, p, q) = E(r, s, q, p) = E(s, r, q, p) = x;
+        } else if (i > 0 && j == 0 && k == 0 && l == 0) {
+            // (i 0 0 0): an orbital energy, written by some programs; recomputed from h and (pq|rs) instead
+        } else {
+            throw std::runtime_error("FCIDUMP line with an invalid index pattern: " + std::to_string(i) + " " +
+                                     std::to_string(j) + " " + std::to_string(k) + " " + std::to_string(l));
         }
-        // (i 0 0 0) lines carry orbital energies in some writers; they are recomputed from h and (pq|rs)
     }
     return d;
 }

@@ -117,7 +117,9 @@ int epilogue_groups(int norb);
 // Packed geometry of each orbit {(A,B,C) and its permutations} for the epilogue: per distinct
 // block d (in the order of the permutations (ABC, BCA, CAB, CBA, ACB, BAC)) its (P,Q,R), the
 // count of distinct blocks and where the five permuted reads of r3 find their Y block.
-void orbit_table(const std::vector<int> &orbits, std::vector<uint64_t> &tab);
+// False when a block index does not fit the table's 10-bit fields (nb > kMaxOrbitBlocks).
+constexpr int kMaxOrbitBlocks = 1024;
+bool orbit_table(const std::vector<int> &orbits, std::vector<uint64_t> &tab);
 hipError_t launch_epilogue(int prec, const EpiArgs &a, int nbatch, hipStream_t s);
 // V pieces of every batch entry from S[p][q][x][y] = (px|qy) into U[nbatch][3][v][vR]
 hipError_t launch_vpieces(const double *S, const int *ijk, int o, int v, int vR, double *U, int nbatch, hipStream_t s);

@@ -942,7 +942,7 @@ int wbuild_tile(int prec, int v) {
 template <int PREC>
 static hipError_t launch_wbuild_p(int blk, bool dma, const WBuildArgs &a, int nbatch, hipStream_t s) {
     const dim3 grid((unsigned)(2 * a.ntile1), (unsigned)nbatch);
-    if (dma && (blk == 2 || blk == 4)) {
+    if (dma && (blk == 2 || blk == 4 || (blk == 5 && PREC != P32))) {
         // two stages: more resident work-groups per CU beat deeper prefetch (sweeps at v = 300)
         int st = 2;
         if (const char *e = std::getenv("SCESS_WBUILD_STAGES"))  // tuning override: 2, 3 or 4
@@ -954,6 +954,9 @@ static hipError_t launch_wbuild_p(int blk, bool dma, const WBuildArgs &a, int nb
     }
         SCESS_DMA_CASE(2, 2) SCESS_DMA_CASE(2, 3) SCESS_DMA_CASE(2, 4)
         SCESS_DMA_CASE(4, 2) SCESS_DMA_CASE(4, 3) SCESS_DMA_CASE(4, 4)
+        if constexpr (PREC != P32) {  // 160-row FP32 chunks do not split into whole wave loads
+            SCESS_DMA_CASE(5, 2) SCESS_DMA_CASE(5, 3)
+        }
 #undef SCESS_DMA_CASE
         return hipErrorInvalidValue;
     }
@@ -1057,22 +1060,22 @@ struct OrbitRow {
 };
 
 // The geometry from the host-built table (orbit_table: one 64-bit entry per orbit and distinct block d):
-// P, Q, R in bits 0-23, the number of distinct blocks in 24-26, and the distinct-block indices of
-// the five permuted reads of r3 in 27-41 (3 bits each): Y(lb, lc, la), Y(lc, la, lb), Y(lc, lb, la),
-// Y(la, lc, lb), Y(lb, la, lc) with lc = (sR, c').
+// P, Q, R in bits 0-29 (10 bits each), the number of distinct blocks in 30-32, and the distinct-block
+// indices of the five permuted reads of r3 in 33-47 (3 bits each): Y(lb, lc, la), Y(lc, la, lb),
+// Y(lc, lb, la), Y(la, lc, lb), Y(lb, la, lc) with lc = (sR, c').
 template <int PB, int PX, int PY>
 __device__ inline OrbitRow orbit_row_tab(const uint64_t *tab, int orb, int tid) {
     constexpr int EB = 8;
     OrbitRow r;
     const int d = tid >> 6, ap = (tid >> 3) & 7, bp = tid & 7;
     const uint64_t e = tab[(int64_t)orb * 6 + d];
-    const int P = (int)(e & 255u), Q = (int)((e >> 8) & 255u), R = (int)((e >> 16) & 255u);
-    r.nd = (int)((e >> 24) & 7u);
+    const int P = (int)(e & 1023u), Q = (int)((e >> 10) & 1023u), R = (int)((e >> 20) & 1023u);
+    r.nd = (int)((e >> 30) & 7u);
     r.own = d < r.nd;
     r.ga = P * EB + ap;
     r.gb = Q * EB + bp;
     r.gc0 = R * EB;
-    auto di = [&](int n) { return (int)((e >> (27 + 3 * n)) & 7u); };
+    auto di = [&](int n) { return (int)((e >> (33 + 3 * n)) & 7u); };
     r.off[0] = di(0) * PB + bp * PX + ap;
     r.step[0] = PY;
     r.off[1] = di(1) * PB + ap * PY + bp;
@@ -1207,9 +1210,12 @@ __global__ __launch_bounds__(EPI_THREADS) void epilogue_kernel(EpiArgs a) {
 
 int epilogue_groups(int norb) { return (norb + EPI_RUN - 1) / EPI_RUN; }
 
-void orbit_table(const std::vector<int> &orbits, std::vector<uint64_t> &tab) {
+bool orbit_table(const std::vector<int> &orbits, std::vector<uint64_t> &tab) {
     const size_t norb = orbits.size() / 3;
     tab.assign(norb * 6, 0);
+    for (int b : orbits)
+        if (b < 0 || b >= kMaxOrbitBlocks)  // block indices are packed into 10 bits
+            return false;
     for (size_t orb = 0; orb < norb; orb++) {
         const int A = orbits[3 * orb], B = orbits[3 * orb + 1], C = orbits[3 * orb + 2];
         const int bl1 = B != A ? B : C;
@@ -1235,19 +1241,20 @@ void orbit_table(const std::vector<int> &orbits, std::vector<uint64_t> &tab) {
         }
         auto dindex = [&](int s0, int s1, int s2) { return key_of[s0 * 9 + s1 * 3 + s2]; };
         for (int d = 0; d < 6; d++) {
-            uint64_t e = (uint64_t)nd << 24;
+            uint64_t e = (uint64_t)nd << 30;
             if (d < nd) {
                 const int sP = sl[d][0], sQ = sl[d][1], sR = sl[d][2];
-                e |= (uint64_t)blk[d][0] | ((uint64_t)blk[d][1] << 8) | ((uint64_t)blk[d][2] << 16);
+                e |= (uint64_t)blk[d][0] | ((uint64_t)blk[d][1] << 10) | ((uint64_t)blk[d][2] << 20);
                 // the five permuted reads of r3: Y(lb,lc,la), Y(lc,la,lb), Y(lc,lb,la), Y(la,lc,lb), Y(lb,la,lc)
                 const int di[5] = {dindex(sQ, sR, sP), dindex(sR, sP, sQ), dindex(sR, sQ, sP), dindex(sP, sR, sQ),
                                    dindex(sQ, sP, sR)};
                 for (int n = 0; n < 5; n++)
-                    e |= (uint64_t)di[n] << (27 + 3 * n);
+                    e |= (uint64_t)di[n] << (33 + 3 * n);
             }
             tab[orb * 6 + d] = e;
         }
     }
+    return true;
 }
 
 hipError_t launch_epilogue(int prec, const EpiArgs &a, int nbatch, hipStream_t s) {

@@ -1,8 +1,9 @@
 """FCIDUMP writer for the command-line runner tests (test helper)."""
 
 
-def write_fcidump(path, h, eri, nelec, core):
-    """FCIDUMP of MO integrals: h (n x n), chemists' (pq|rs) (C order), each unique element once."""
+def write_fcidump(path, h, eri, nelec, core, orbital_energies=None):
+    """FCIDUMP of MO integrals: h (n x n), chemists' (pq|rs) (C order), each unique element once; with
+    orbital_energies, also the `e_p p 0 0 0` lines some programs write after the one-electron block."""
     n = h.shape[0]
     with open(path, "w") as f:
         f.write(f" &FCI NORB={n},NELEC={nelec},MS2=0,\n  ORBSYM={','.join(['1'] * n)},\n  ISYM=1,\n &END\n")
@@ -15,4 +16,7 @@ def write_fcidump(path, h, eri, nelec, core):
         for p in range(n):
             for q in range(p + 1):
                 f.write(f"{h[p, q]:.17e} {p + 1} {q + 1} 0 0\n")
+        if orbital_energies is not None:
+            for p in range(n):
+                f.write(f"{orbital_energies[p]:.17e} {p + 1} 0 0 0\n")
         f.write(f"{core:.17e} 0 0 0 0\n")

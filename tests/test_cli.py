@@ -33,6 +33,27 @@ def test_cli_hf_from_fcidump(tmp_path):
 
 
 @pytest.mark.skipif(not os.path.exists(CLI), reason="scess_cc not built")
+def test_cli_hf_fcidump_with_orbital_energy_lines(tmp_path):
+    """`e_p p 0 0 0` lines (written by some programs) are skipped: the run equals the one without them.
+    A line with any other zero pattern (here `x 1 0 2 0`) is an error, not a silent write."""
+    from oracle import scf
+    atoms = scf.water_builtin()
+    S, H, G = scf.ao_integrals("STO-3G", atoms)
+    E_HF, e, C = scf.rhf(S, H, G, 5, scf.nuclear_repulsion(atoms))
+    plain, with_e = tmp_path / "plain", tmp_path / "with_e"
+    write_fcidump(plain, C.T @ H @ C, scf.mo_eri(G, C), 10, scf.nuclear_repulsion(atoms))
+    write_fcidump(with_e, C.T @ H @ C, scf.mo_eri(G, C), 10, scf.nuclear_repulsion(atoms), orbital_energies=e + 7.0)
+    r0 = subprocess.run([CLI, "hf", str(plain)], capture_output=True, text=True, timeout=60)
+    r1 = subprocess.run([CLI, "hf", str(with_e)], capture_output=True, text=True, timeout=60)
+    assert r0.returncode == 0 and r1.returncode == 0, r1.stderr
+    assert r1.stdout == r0.stdout
+    bad = tmp_path / "bad"
+    bad.write_text(with_e.read_text() + "1.0 1 0 2 0\n")
+    r2 = subprocess.run([CLI, "hf", str(bad)], capture_output=True, text=True, timeout=60)
+    assert r2.returncode == 1 and "invalid index pattern" in r2.stderr
+
+
+@pytest.mark.skipif(not os.path.exists(CLI), reason="scess_cc not built")
 def test_cli_rejects_bad_usage(tmp_path):
     assert subprocess.run([CLI], capture_output=True).returncode == 1
     assert subprocess.run([CLI, "fci", "x"], capture_output=True).returncode == 1

@@ -152,3 +152,49 @@ def test_product_ijk_enumeration_matches_oracle():
         assert np.array_equal(unique_ijk(o), t)
         w = ijk_weights(o)
         assert w.sum() == 2 * (o * (o - 1) * (o - 2) // 6) + 2 * o * (o - 1) // 2  # 2/deg per unique ijk
+
+
+def test_grad_items_partition_and_local_differences():
+    """oracle.t_grad(items=...) is the gradient of sum over the listed triples of 2/deg F_ijk: over a
+    partition of the triples the parts add up to the full gradient (1e-12 relative), and one part
+    matches central differences of the same restricted sum (E is quadratic per entry)."""
+    from scess_amd.triples import ijk_weights
+    inp = make_triples_inputs(4, 6, 17)
+    full = oracle.t_grad(inp)
+    n = len(oracle.unique_ijk(4))
+    part_a = np.arange(0, n, 2)
+    part_b = np.arange(1, n, 2)
+    ga, gb = oracle.t_grad(inp, items=part_a), oracle.t_grad(inp, items=part_b)
+    for k in full:
+        scale = np.abs(full[k]).max()
+        assert np.abs(ga[k] + gb[k] - full[k]).max() <= 1e-12 * scale, k
+        assert np.abs(ga[k]).max() > 0.0
+    w = ijk_weights(4)[part_a]
+    rng = np.random.default_rng(3)
+    h = 1e-3
+    for name in ("ovvv", "ovoo", "t2", "ovov", "t1"):
+        arr = getattr(inp, name)
+        ix = np.unravel_index(int(rng.integers(arr.size)), arr.shape, order="F")
+        vals = []
+        for sgn in (1.0, -1.0):
+            a = arr.copy(order="F")
+            a[ix] += sgn * h
+            vals.append(float(np.dot(w, oracle.t_ijk(_replace(inp, **{name: a}), items=part_a)[1])))
+        fd = (vals[0] - vals[1]) / (2 * h)
+        assert abs(fd - ga[name][ix]) <= 1e-7 * max(1.0, abs(ga[name][ix])), (name, ix, fd, ga[name][ix])
+
+
+def test_blas_ijk_restatement_matches_oracle():
+    """oracle.triples_blas.t_ijk_blas (dense BLAS form used for single triples at large v) equals the C
+    oracle's ijk-driven F_ijk, 1e-12 relative, on triples with distinct and with two equal indices."""
+    from oracle.triples_blas import t_ijk_blas
+    from scess_amd.triples import unique_ijk
+    for (o, v, seed) in ((3, 5, 1), (5, 13, 2), (4, 24, 3)):
+        inp = make_triples_inputs(o, v, seed)
+        t = unique_ijk(o)
+        _, F = oracle.t_ijk(inp)
+        for n, (i, j, k) in enumerate(t):
+            if i == k:
+                continue
+            Fb = t_ijk_blas(inp, int(i), int(j), int(k))
+            assert abs(Fb - F[n]) <= 1e-12 * max(abs(F[n]), 1e-300), (o, v, (i, j, k), Fb, F[n])
