@@ -228,7 +228,34 @@ RCCSD_Wfn RCCSD_T(const HF::HF_Wfn &rhfWfn, const Integrals::MolInts &mints, con
 RCCSD_Wfn RCCSD_DIIS(const HF::HF_Wfn &rhfWfn, const Integrals::MolInts &mints, const Thread *thread = nullptr,
                      std::stringstream *console = nullptr);
 
+// Spin-orbital CCSD and CCSD(T) (src/Quantum/CC/CC.h:27-33,57-73; CCSD.cpp, CCSD_T.cpp): the
+// reference's independent implementations in (2n)^4 antisymmetrised integrals, kept to test the
+// closed-shell solvers. Host code, tiny systems only (the regression runner's H2 and water STO-3G).
+struct CCSD_Wfn {
+    double E_CCSD = 0.0;
+    double E_T = 0.0;
+    bool converged = false;
+};
+CCSD_Wfn CCSD_DIIS(const HF::HF_Wfn &rhfWfn, const Integrals::MolInts &mints);
+CCSD_Wfn CCSD_T(const HF::HF_Wfn &rhfWfn, const Integrals::MolInts &mints);
+// The same from MO integrals: eri is <pq|rs> (spatial, occupied first), E_T only when withTriples.
+CCSD_Wfn CCSD_DIIS_FromMO(const TEITensor &eri, const std::vector<double> &orbitalEnergies, int64_t nOcc,
+                          bool withTriples);
+
 }  // namespace CC
+
+namespace HF {
+// MP2_RHF_ClosedShellAO (src/Quantum/HF/HF.h:148-151): the closed-shell MP2 correlation energy from the
+// AO integrals (transformed with TransformTEI on the GPU).
+double MP2_RHF_ClosedShellAO(const HF_Wfn &wfn, const Integrals::TEITensor &aoERI);
+}  // namespace HF
+
+namespace Tests {
+// RunRegressionTests (src/SCESS/Tests.cpp:494-737, `SCESS test`): the closed-shell CC rows, each printed
+// as [PASS] / [FAIL] with value, reference, |diff| and tolerance; returns the number of failures.
+// Basis sets are looked up as Basis::LoadBasisSet does ($SCESS_BASIS_DIR).
+int RunRegressionTests();
+}  // namespace Tests
 
 namespace Driver {
 // src/SCESS/Driver.h:13-40, Driver.cpp:51-65,173-231: RHF on the basis set (a BSE JSON name or path, as

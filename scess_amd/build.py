@@ -16,8 +16,9 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libscess_gpu.so")
 CLI = os.path.join(HERE, "scess_cc")
 CPP_CHECK = os.path.join(ROOT, "tests", "cpp", "triples_function_check")
+SO_CHECK = os.path.join(ROOT, "tests", "cpp", "spin_orbital_check")
 SOURCES = ["triples_kernels.hip", "triples_adjoint.hip", "ccsd_engine.hip", "scess_capi.hip", "cc_api.cpp",
-           "scf_host.cpp"]
+           "scf_host.cpp", "spin_orbital_cc.cpp", "regression.cpp"]
 ARCH = "gfx950"
 
 # Packed FP32 instructions are disabled for all device code.
@@ -64,6 +65,10 @@ def build(verbose: bool = False) -> str:
     _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), "-o", CPP_CHECK + ".tmp",
           CPP_CHECK + ".cpp", "-L", HERE, "-lscess_gpu", "-Wl,-rpath,$ORIGIN/../../scess_amd"])
     os.replace(CPP_CHECK + ".tmp", CPP_CHECK)
+    # host-only check of the spin-orbital CCSD(T) (tests/test_spin_orbital_cpu.py)
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), "-o", SO_CHECK + ".tmp",
+          SO_CHECK + ".cpp", "-L", HERE, "-lscess_gpu", "-Wl,-rpath,$ORIGIN/../../scess_amd"])
+    os.replace(SO_CHECK + ".tmp", SO_CHECK)
     return LIB
 
 

@@ -2,6 +2,7 @@
 //
 //   scess_cc rhf|ccsd|ccsd_t <molecule> <basis>   as the reference's `SCESS ccsd_t H2O cc-pVDZ`
 //   scess_cc ccsd|ccsd_t|hf  <FCIDUMP>            from canonical RHF molecular orbitals
+//   scess_cc test                                 the regression runner, as the reference's `SCESS test`
 //
 // The first form follows the reference's CLI (src/SCESS/Application.cpp:23-277) and drivers
 // (Driver.cpp:51-65, 173-231): a built-in molecule (H2, LiH, NH3, H2O), a BSE-JSON basis set found
@@ -100,6 +101,11 @@ int run_molecule(const std::string &method, const std::string &molName, const st
 }  // namespace
 
 int main(int argc, char **argv) {
+    if (argc == 2 && std::string(argv[1]) == "test") {
+        // the reference's `SCESS test` (Application.cpp:49-58): exit code 0 only when every check passes
+        const int failures = SCESS::Tests::RunRegressionTests();
+        return failures == 0 ? 0 : 1;
+    }
     if (argc == 4) {
         std::string method = argv[1];
         for (char &c : method)
@@ -117,8 +123,9 @@ int main(int argc, char **argv) {
     }
     const std::string method = argc == 3 ? argv[1] : "";
     if (method != "ccsd" && method != "ccsd_t" && method != "hf") {
-        std::fprintf(stderr, "usage: %s rhf|ccsd|ccsd_t <molecule> <basis>\n       %s ccsd|ccsd_t|hf <FCIDUMP>\n",
-                     argv[0], argv[0]);
+        std::fprintf(stderr,
+                     "usage: %s rhf|ccsd|ccsd_t <molecule> <basis>\n       %s ccsd|ccsd_t|hf <FCIDUMP>\n       %s test\n",
+                     argv[0], argv[0], argv[0]);
         return 1;
     }
     const bool triples = method == "ccsd_t";

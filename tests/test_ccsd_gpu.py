@@ -291,3 +291,28 @@ def test_mixed_precision_contractions(ctx, mode, tol):
         assert cm["converged"] and abs(cm["E_CCSD"] - c64["E_CCSD"]) < 1e-6
     finally:
         ctx.set_ccsd_precision("fp64")
+
+
+def test_cli_regression_runner(tmp_path):
+    """`scess_cc test` as the reference's `SCESS test` (Application.cpp:49-58, Tests.cpp:494-737): the CC
+    rows print [PASS] lines with the reference's tolerances (CheckEnergy, Tests.cpp:77-91), including the
+    closed-shell vs spin-orbital (T) and CCSD cross-checks, and the exit code is 0 only when all pass.
+    With one basis set missing, its sections fail and the exit code is 1."""
+    import os
+    import subprocess
+    from test_cli import make_basis_dir
+    _say("case cli regression runner")
+    cli = os.path.join(os.path.dirname(gpu.LIB_PATH), "scess_cc")
+    basis_dir = make_basis_dir(tmp_path)
+    r = subprocess.run([cli, "test"], capture_output=True, text=True, timeout=300,
+                       env=dict(os.environ, SCESS_BASIS_DIR=basis_dir))
+    _say(r.stdout[-2500:], r.stderr[-600:])
+    lines = [l for l in r.stdout.splitlines() if l.strip().startswith("[")]
+    assert r.returncode == 0
+    assert len(lines) == 16 and all(l.strip().startswith("[PASS]") for l in lines)
+    assert "closed-shell vs spin orbitals" in r.stdout and "spatial vs spin orbitals" in r.stdout
+    assert "All regression tests passed" in r.stdout
+    os.remove(os.path.join(basis_dir, "Pople", "6-31G.json"))
+    r = subprocess.run([cli, "test"], capture_output=True, text=True, timeout=300,
+                       env=dict(os.environ, SCESS_BASIS_DIR=basis_dir))
+    assert r.returncode == 1 and "[FAIL] H2O / 6-31G" in r.stdout and "1 regression test(s) failed" in r.stdout
