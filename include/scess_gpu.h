@@ -119,6 +119,14 @@ int scess_t_forward(scess_ctx *ctx, int prec, double *e_t, scess_t_stats *stats)
 int scess_t_forward_ex(scess_ctx *ctx, int prec, double *e_t, double *F_ijk, double *partials,
                        scess_t_stats *stats);
 int64_t scess_t_num_ijk(const scess_ctx *ctx);
+/* E(T) from the resident inputs without allocating device memory (no counterpart in the reference).
+ * For callers that cannot call hipMalloc (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid allocation): W, the V pieces, partials, F_ijk and E(T) live in a 64 MiB device array of
+ * the library (one per GPU, held until the process ends), one occupied triple per launch on one
+ * stream. Sizes whose scratch does not fit (mostly W: 2 vR^3 bytes-per-element, vR = v rounded up to
+ * 16; FP64 and OZAKI up to v = 160, FP32 and BF16X3 up to v = 192) are rejected with SCESS_ERR_ARG. The result equals scess_t_forward's bit for bit.
+ * Every call on a GPU shares that GPU's array: calls that overlap on one GPU race. */
+int scess_t_forward_noalloc(scess_ctx *ctx, int prec, double *e_t);
 /* Restrict scess_t_forward / scess_t_backward to one of nshards contiguous, equal-count ranges of the
  * occupied-triple work list (no counterpart in the reference; default 0 of 1 = everything). The
  * result is that shard's share: the sum over shards of E(T), F_ijk and the gradients is the full

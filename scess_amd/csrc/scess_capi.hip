@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -53,6 +54,7 @@ struct scess_ctx {
     int64_t nbatch_req = 0;
     int64_t shard = 0, nshards = 1;  // (T) evaluates work items [n*shard/nshards, n*(shard+1)/nshards)
     int ccsd_prec = 0;               // product precision of the CCSD residual contractions
+    void *scratch = nullptr;         // this device's address of the static scratch array
     TriplesState t;
     scess::Engine eng;
     scess::CcsdState cc;
@@ -126,6 +128,33 @@ int upload(scess_ctx *c, const double *h, size_t n, double **d) {
     return SCESS_OK;
 }
 
+// Device scratch of scess_t_forward_noalloc: W of both families, V pieces and epilogue partials of one
+// occupied triple, F_ijk and E(T). Every GPU the library runs on holds its own copy until the process
+// ends. Host code never uses the name `scratch` as an address: each device's address comes from
+// hipGetSymbolAddress, taken once per device in scess_create with that device current.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device` (current), resolved once per device.
+hipError_t scratch_address(int device, void **out) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (device < 0 || device >= kMaxDevices)
+        return hipErrorInvalidDevice;
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess)
+            return e;
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
+
 }  // namespace
 
 extern "C" {
@@ -151,6 +180,12 @@ int scess_create(scess_ctx **out, int device) {
         return SCESS_ERR_HIP;
     }
     c->eng.stream = c->stream;
+    if (scratch_address(device, &c->scratch) != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        (void)hipStreamDestroy(c->stream2);
+        delete c;
+        return SCESS_ERR_HIP;
+    }
     *out = c;
     return SCESS_OK;
 }
@@ -716,6 +751,102 @@ int scess_t_forward_ex(scess_ctx *c, int prec, double *e_t, double *F_ijk, doubl
     return rc;
 }
 
+
+int scess_t_forward_noalloc(scess_ctx *c, int prec, double *e_t) {
+    if (!c)
+        return SCESS_ERR_ARG;
+    if (!e_t)
+        return fail(c, SCESS_ERR_ARG, "scess_t_forward_noalloc: e_t is null");
+    *e_t = 0.0;
+    if (prec < 0 || prec >= kNumPrec)
+        return fail(c, SCESS_ERR_ARG, "scess_t_forward_noalloc: unknown precision");
+    TriplesState &t = c->t;
+    if (!t.have)
+        return fail(c, SCESS_ERR_STATE, "scess_t_forward_noalloc: no inputs (call scess_t_set_inputs first)");
+    if (!t.have_prec[prec])
+        return fail(c, SCESS_ERR_STATE, "scess_t_forward_noalloc: precision not packed by scess_t_set_inputs");
+    if (!c->scratch)
+        return fail(c, SCESS_ERR_STATE, "scess_t_forward_noalloc: no scratch address for this device");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const int64_t o = t.o, v = t.v, vR = t.vR;
+    const int64_t nall = (int64_t)t.ijk_all.size() / 3;
+    const int64_t nwork = (int64_t)t.work.size();
+    const int64_t w_begin = nwork * c->shard / c->nshards, w_end = nwork * (c->shard + 1) / c->nshards;
+    const size_t accb = prec == P64 || prec == PI8 ? 8 : 4;
+    const int64_t wstride = vR * vR * vR;
+    // scratch layout (256-byte aligned pieces): W1, W2 [vR^3 accb], U [3 v vR], partials [ngrp], F [nall], E
+    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t wb = al((size_t)wstride * accb), ub = al(sizeof(double) * 3 * (size_t)(v * vR)),
+                 pb = al(sizeof(double) * (size_t)t.ngrp), fb = al(sizeof(double) * (size_t)nall);
+    const size_t need = 2 * wb + ub + pb + fb + 256;
+    if (need > kNoAllocScratchBytes)
+        return fail(c, SCESS_ERR_ARG,
+                    "scess_t_forward_noalloc: o=" + std::to_string(o) + ", v=" + std::to_string(v) + " needs " +
+                        std::to_string(need) + " bytes of scratch, more than its " +
+                        std::to_string(kNoAllocScratchBytes) + " (use scess_t_forward)");
+    char *base = (char *)c->scratch;
+    void *W1 = base, *W2 = base + wb;
+    double *U = (double *)(base + 2 * wb), *partials = (double *)(base + 2 * wb + ub),
+           *F = (double *)(base + 2 * wb + ub + pb), *E = (double *)(base + 2 * wb + ub + pb + fb);
+
+    const int blk = wbuild_tile(prec, (int)v);
+    WBuildArgs wa = wbuild_args(t, prec, blk, wstride);
+    wa.W1 = W1;
+    wa.W2 = W2;
+    EpiArgs ea{};
+    ea.W1 = W1;
+    ea.W2 = W2;
+    ea.wstride = wstride;
+    ea.o = (int)o;
+    ea.v = (int)v;
+    ea.vR = (int)vR;
+    ea.nb = t.nb;
+    ea.orbits = t.d_orbits;
+    ea.orbtab = t.d_orbtab;
+    ea.norb = t.norb;
+    ea.ngrp = t.ngrp;
+    ea.t1 = t.t1;
+    ea.e = t.e;
+    ea.ev = t.ev;
+    ea.U = U;
+    ea.partials = partials;
+    ea.halve = 1;
+
+    hipStream_t s = c->stream;
+    hipError_t he = launch_fill_zero(F, nall, s);
+    bool cancelled = false;
+    for (int64_t w = w_begin; w < w_end && he == hipSuccess; w++) {
+        if (c->should_stop && ((w - w_begin) & 63) == 0) {
+            if ((he = hipStreamSynchronize(s)) != hipSuccess)
+                break;
+            if (c->should_stop(c->user)) {
+                cancelled = true;
+                break;
+            }
+        }
+        wa.ijk = t.d_work_ijk + 3 * w;
+        ea.ijk = wa.ijk;
+        he = launch_wbuild(prec, blk, wbuild_dma(), wa, 1, s);
+        if (he == hipSuccess)
+            he = launch_vpieces(t.S, ea.ijk, (int)o, (int)v, (int)vR, U, 1, s);
+        if (he == hipSuccess)
+            he = launch_epilogue(prec, ea, 1, s);
+        if (he == hipSuccess)
+            he = launch_reduce_partials(partials, t.ngrp, t.d_work_index + w, F, 1, s);
+    }
+    if (he == hipSuccess && !cancelled)
+        he = launch_reduce_energy(F, t.d_ijk_all, (int)nall, E, s);
+    if (he == hipSuccess)
+        he = hipStreamSynchronize(s);
+    if (he != hipSuccess)
+        return hip_fail(c, he, "scess_t_forward_noalloc");
+    if (cancelled)
+        return fail(c, SCESS_CANCELLED, "scess_t_forward_noalloc: cancelled");
+    double Eh = 0.0;
+    HIP_TRY(c, hipMemcpy(&Eh, E, sizeof(double), hipMemcpyDeviceToHost));
+    *e_t = Eh;
+    return SCESS_OK;
+}
 
 int scess_t_backward(scess_ctx *c, int prec, double dE, scess_t_grad *grad) {
     if (!c)

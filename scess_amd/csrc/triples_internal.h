@@ -129,6 +129,8 @@ hipError_t launch_reduce_partials(const double *partials, int norb, const int *i
                                   hipStream_t s);
 // out[0] = sum_t 2/deg_t F[t] over the unique occupied triples (deterministic, compensated)
 hipError_t launch_reduce_energy(const double *F, const int *ijk_all, int nall, double *out, hipStream_t s);
+// p[0..n) = 0 by a kernel (usable where a hipMemsetAsync node is not wanted)
+hipError_t launch_fill_zero(double *p, int64_t n, hipStream_t s);
 
 // ----------------------------------------------------------------- adjoint
 // FP64 operand of the generic GEMM: element (r, k1, k2) of batch b at

@@ -1298,6 +1298,19 @@ __global__ __launch_bounds__(256) void reduce_energy_kernel(const double *F, con
         out[0] = tot;
 }
 
+__global__ __launch_bounds__(256) void fill_zero_kernel(double *p, int64_t n) {
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256)
+        p[t] = 0.0;
+}
+
+hipError_t launch_fill_zero(double *p, int64_t n, hipStream_t s) {
+    if (n <= 0)
+        return hipSuccess;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 1024);
+    hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_reduce_energy(const double *F, const int *ijk_all, int nall, double *out, hipStream_t s) {
     hipLaunchKernelGGL(reduce_energy_kernel, dim3(1), dim3(256), 0, s, F, ijk_all, nall, out);
     return hipGetLastError();

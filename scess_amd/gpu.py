@@ -109,6 +109,8 @@ def load(path: str = LIB_PATH):
     lib.scess_t_num_ijk.restype = c.c_int64
     lib.scess_t_num_partials.argtypes = [c.c_void_p]
     lib.scess_t_num_partials.restype = c.c_int64
+    lib.scess_t_forward_noalloc.argtypes = [c.c_void_p, c.c_int, _dp]
+    lib.scess_t_forward_noalloc.restype = c.c_int
     lib.scess_t_backward.argtypes = [c.c_void_p, c.c_int, c.c_double, c.POINTER(TGrad)]
     lib.scess_t_backward.restype = c.c_int
     lib.scess_ccsd_set_inputs.argtypes = [c.c_void_p, c.POINTER(CcsdIn)]
