@@ -117,7 +117,7 @@ struct TriplesGradient {
 // and keeps the inputs resident on the GPU; its contractions can also run in FP32, split-BF16x3 or
 // exact int8-digit products (OZAKI), with FP64 denominators and sums. backward(dE) returns dE times
 // the gradients of E(T), W being recomputed per occupied triple in the precision given to the
-// constructor (FP64, FP32 or BF16X3; FP64 by default), the back-contractions in FP64. backward()
+// constructor (FP64, FP32, BF16X3 or OZAKI; FP64 by default), the back-contractions in FP64. backward()
 // before a successful forward() throws; a cancelled backward() returns an empty TriplesGradient.
 // One object holds one GPU context; it is not copyable.
 class TriplesFunction {

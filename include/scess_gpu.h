@@ -50,9 +50,10 @@ enum {
     SCESS_PREC_FP64 = 0,   /* FP64 operands, FP64 MFMA */
     SCESS_PREC_FP32 = 1,   /* FP32 operands, FP32 MFMA, FP32 accumulation */
     SCESS_PREC_BF16X3 = 2, /* x = hi + lo in bf16; hi*hi + hi*lo + lo*hi on bf16 MFMA, FP32 accumulation */
-    SCESS_PREC_OZAKI = 3   /* (T) forward only: each tensor scaled by a power of two and split into 7 int8 digits
-                              (Ozaki); the 28 leading digit products summed exactly on int8 MFMA, FP64
-                              accumulation; product error <= ~2^-46 of max|A| max|B| */
+    SCESS_PREC_OZAKI = 3   /* each tensor scaled by a power of two and split into S signed 8-bit
+                              digits (Ozaki; S = 5, 6 or 7, scess_t_set_int8_digits); the S(S+1)/2 leading
+                              digit products summed exactly on int8 MFMA, FP64 accumulation; product error
+                              <= ~(S+1) 2^-8S of max|A| max|B| */
 };
 
 /* Inputs of the (T) functional. Replaces the (eri, orbitalEnergies, nOcc, t1,
@@ -105,6 +106,9 @@ int scess_set_batch(scess_ctx *ctx, int64_t nbatch);
  * they stay resident in HBM for any number of forward/backward calls. This is
  * the device counterpart of the serial repack of RCCSD_T.cpp:27-56. */
 int scess_t_set_inputs(scess_ctx *ctx, const scess_t_in *in, int prec_mask);
+/* Digits per element of SCESS_PREC_OZAKI (5, 6 or 7; default 6), applied by the next
+ * scess_t_set_inputs (no counterpart in the reference). */
+int scess_t_set_int8_digits(scess_ctx *ctx, int digits);
 
 /* E(T) (RCCSD_TriplesCorrection, src/Quantum/CC/RCCSD_T.cpp:17-168) from the
  * resident inputs. stats may be NULL. */
@@ -138,7 +142,7 @@ int64_t scess_t_num_partials(const scess_ctx *ctx);
  * for every input entry, in the input layouts, into the caller's host arrays
  * (overwritten). Every array entry is an independent variable read exactly as
  * the forward reads it. W is recomputed per occupied triple in precision prec
- * (FP64, FP32 or BF16X3; SCESS_PREC_OZAKI is rejected); the back-contractions run in FP64 and need SCESS_PREC_FP64 packed by
+ * (FP64, FP32, BF16X3 or OZAKI); the back-contractions run in FP64 and need SCESS_PREC_FP64 packed by
  * scess_t_set_inputs. Deterministic: triples are processed in a fixed order. */
 int scess_t_backward(scess_ctx *ctx, int prec, double dE, scess_t_grad *grad);
 

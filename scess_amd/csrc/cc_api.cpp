@@ -233,10 +233,6 @@ int prec_code(TriplesFunction::Precision p) {
 }  // namespace
 
 TriplesFunction::TriplesFunction(Precision wRecompute, int device) : s_(new State) {
-    if (wRecompute == Precision::OZAKI) {
-        delete s_;
-        throw std::runtime_error("SCESS: TriplesFunction: the W recomputation of the backward takes FP64, FP32 or BF16X3");
-    }
     s_->prec = prec_code(wRecompute);
     if (scess_create(&s_->c, device) != SCESS_OK) {
         delete s_;

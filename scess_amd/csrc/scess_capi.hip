@@ -55,6 +55,7 @@ struct scess_ctx {
     int64_t shard = 0, nshards = 1;  // (T) evaluates work items [n*shard/nshards, n*(shard+1)/nshards)
     int ccsd_prec = 0;               // product precision of the CCSD residual contractions
     void *scratch = nullptr;         // this device's address of the static scratch array
+    int int8_digits = kI8DefaultDigits;  // digits per element of SCESS_PREC_OZAKI (next scess_t_set_inputs)
     TriplesState t;
     scess::Engine eng;
     scess::CcsdState cc;
@@ -231,6 +232,15 @@ int scess_t_set_shard(scess_ctx *c, int64_t shard, int64_t nshards) {
     return SCESS_OK;
 }
 
+int scess_t_set_int8_digits(scess_ctx *c, int digits) {
+    if (!c)
+        return SCESS_ERR_ARG;
+    if (digits < kI8MinDigits || digits > kI8MaxDigits)
+        return fail(c, SCESS_ERR_ARG, "scess_t_set_int8_digits: digits must be 5, 6 or 7");
+    c->int8_digits = digits;
+    return SCESS_OK;
+}
+
 int64_t scess_t_num_ijk(const scess_ctx *c) { return c && c->t.have ? (int64_t)c->t.ijk_all.size() / 3 : 0; }
 
 int64_t scess_t_num_partials(const scess_ctx *c) { return c && c->t.have ? c->t.ngrp : 0; }
@@ -294,7 +304,7 @@ int scess_t_set_inputs(scess_ctx *c, const scess_t_in *in, int prec_mask) {
             return hip_fail(c, e, "pack kernel");
         return SCESS_OK;
     };
-    // PI8: one power-of-two scale per tensor (|x| 2^-e < 1/2), then the digit slices
+    // PI8: one power-of-two scale per tensor (|x| 2^-e <= 0.498), then the digit slices
     auto pack_i8 = [&](Operands &op, const double *ovvv, const double *t2, const double *ovoo) -> int {
         unsigned long long *dmax = nullptr;
         hipError_t e = hipMalloc((void **)&dmax, 3 * sizeof(unsigned long long));
@@ -321,15 +331,17 @@ int scess_t_set_inputs(scess_ctx *c, const scess_t_in *in, int prec_mask) {
             std::memcpy(&mx, &hmax[n], sizeof(mx));
             if (!std::isfinite(mx))
                 return fail(c, SCESS_ERR_ARG, "scess_t_set_inputs: non-finite input");
+            // the smallest e with mx 2^-e <= 0.498 (mx = f 2^E, f in [1/2, 1): e = E + 1 unless f > 0.996)
             int E = 0;
-            (void)std::frexp(mx, &E);  // mx < 2^E
-            ex[n] = mx > 0.0 ? E + 1 : 0;
+            const double f = std::frexp(mx, &E);
+            ex[n] = mx > 0.0 ? (0.5 * f <= 0.498 ? E + 1 : E + 2) : 0;
         }
-        op.eX = ex[0];
+        // one scale for both integral blocks, so that the f- and m-term digit products of a W element
+        // share their int32 groups (the smaller block loses its log2 of the max ratio in digits)
+        op.eX = op.eR = std::max(ex[0], ex[2]);
         op.eT = ex[1];
-        op.eR = ex[2];
-        // int32 groups: |digit product| <= 4096, at most kI8Slices pairs per group, 4 vK + 4 oK deep
-        if (4096.0 * kI8Slices * (4.0 * (double)op.vK + 4.0 * (double)op.oK) >= 2147483647.0)
+        // int32 groups: |digit product| <= 2^14, at most `slices` pairs per group, 4 vK + 4 oK deep
+        if (16384.0 * op.slices * (4.0 * (double)op.vK + 4.0 * (double)op.oK) >= 2147483647.0)
             return fail(c, SCESS_ERR_ARG, "scess_t_set_inputs: too large for the int8 mode");
         struct Job {
             int mode;
@@ -342,12 +354,12 @@ int scess_t_set_inputs(scess_ctx *c, const scess_t_in *in, int prec_mask) {
                      {PACK_RN, ovoo, &op.Rn, o * o * v, op.oK, 32, op.eR},
                      {PACK_TM, t2, &op.TM, o * v * v, op.oK, 32, op.eT}};
         for (const Job &jb : jobs) {
-            const size_t bytes = (size_t)jb.rows * (size_t)jb.kpad * kI8Slices;
+            const size_t bytes = (size_t)jb.rows * (size_t)jb.kpad * (size_t)op.slices;
             e = hipMalloc(jb.dst, std::max<size_t>(bytes, 16));
             if (e != hipSuccess)
                 return hip_fail(c, e, "hipMalloc (packed operand)");
             e = launch_pack_i8(jb.mode, jb.src, *jb.dst, (int)o, (int)v, jb.kpad, jb.gran, std::ldexp(1.0, -jb.e),
-                               c->stream);
+                               op.slices, c->stream);
             if (e != hipSuccess)
                 return hip_fail(c, e, "pack_i8 kernel");
         }
@@ -371,8 +383,9 @@ int scess_t_set_inputs(scess_ctx *c, const scess_t_in *in, int prec_mask) {
         Operands &op = t.ops[p];
         op.vK = round_up(v, kChunk[p]);
         op.oK = round_up(o, kChunkO[p]);
-        op.rbv = op.vK * kElemBytes[p];
-        op.rbo = op.oK * kElemBytes[p];
+        op.slices = p == PI8 ? c->int8_digits : 0;
+        op.rbv = op.vK * kElemBytes[p] * (p == PI8 ? op.slices : 1);
+        op.rbo = op.oK * kElemBytes[p] * (p == PI8 ? op.slices : 1);
         if (p == PI8) {
             if ((rc = pack_i8(op, r_ovvv, r_t2, r_ovoo))) {
                 cleanup();
@@ -485,9 +498,9 @@ static WBuildArgs wbuild_args(const TriplesState &t, int prec, int blk, int64_t 
     wa.ntile1 = (int)(v * wa.nt * wa.nt);
     wa.wstride = wstride;
     wa.halve = 1;
-    wa.scale_f = std::ldexp(1.0, op.eX + op.eT);
-    wa.scale_m = std::ldexp(1.0, op.eR + op.eT);
+    wa.scale = std::ldexp(1.0, op.eX + op.eT);
     wa.zero = t.zero;
+    wa.slices = op.slices;
     return wa;
 }
 
@@ -855,8 +868,6 @@ int scess_t_backward(scess_ctx *c, int prec, double dE, scess_t_grad *grad) {
         return fail(c, SCESS_ERR_ARG, "scess_t_backward: null gradient array");
     if (prec < 0 || prec >= kNumPrec)
         return fail(c, SCESS_ERR_ARG, "scess_t_backward: unknown precision");
-    if (prec == PI8)  // switched off: a run of this path at o = 34, v = 7 faulted the GPU, cause not found
-        return fail(c, SCESS_ERR_ARG, "scess_t_backward: SCESS_PREC_OZAKI is forward only");
     TriplesState &t = c->t;
     if (!t.have)
         return fail(c, SCESS_ERR_STATE, "scess_t_backward: no inputs (call scess_t_set_inputs first)");
@@ -899,7 +910,10 @@ int scess_t_backward(scess_ctx *c, int prec, double dE, scess_t_grad *grad) {
     const size_t nX = (size_t)(o * v * v * vK), nT = (size_t)(o * o * v * vK), nR = (size_t)(o * o * v * oK),
                  nTM = (size_t)(o * v * v * oK), nS = (size_t)(o * o * v * v), nt1 = (size_t)(o * vR),
                  nG = (size_t)wstride, nscr = (size_t)std::max<int64_t>(PT * v * v, PR * v * o);
-    const size_t nout = std::max<size_t>({(size_t)(o * v * v * v), (size_t)(o * o * v * v), (size_t)(o * v * o * v)});
+    // staging of each unpacked gradient before its download: the largest of ovvv (o v^3), ovoo (o^3 v),
+    // ovov and t2 (o^2 v^2) and t1 (o v)
+    const size_t nout = std::max<size_t>({(size_t)(o * v * v * v), (size_t)(o * v * o * o), (size_t)(o * o * v * v),
+                                          (size_t)(o * v)});
     if ((he = alloc(8 * nX, (void **)&dX)) != hipSuccess || (he = alloc(8 * nT, (void **)&dT)) != hipSuccess ||
         (he = alloc(8 * nR, (void **)&dRn)) != hipSuccess || (he = alloc(8 * nTM, (void **)&dTM)) != hipSuccess ||
         (he = alloc(8 * nS, (void **)&dS)) != hipSuccess || (he = alloc(8 * nt1, (void **)&dt1)) != hipSuccess ||

@@ -17,9 +17,11 @@ constexpr int kNumPrec = 4;
 // share one 64-deep chunk).
 constexpr int kChunk[kNumPrec] = {16, 16, 32, 64};
 constexpr int kChunkO[kNumPrec] = {16, 16, 32, 32};
-// Bytes of one packed element: FP64 8, FP32 4, BF16X3 4 (hi and lo bf16), PI8 one per slice.
-constexpr int kI8Slices = 7;
-constexpr int kElemBytes[kNumPrec] = {8, 4, 4, kI8Slices};
+// Bytes of one packed element: FP64 8, FP32 4, BF16X3 4 (hi and lo bf16); PI8 one per digit (the
+// digit count is chosen per context, Operands::slices).
+constexpr int kElemBytes[kNumPrec] = {8, 4, 4, 1};
+// PI8 digits per element: 5, 6 or 7 (15, 21 or 28 int8 digit products per FP64 product)
+constexpr int kI8MinDigits = 5, kI8MaxDigits = 7, kI8DefaultDigits = 6;
 
 inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
@@ -49,15 +51,17 @@ __device__ inline int xcd_remap(int w, int n) {
 // FP64 and FP32 store plain rows; BF16X3 stores each row as groups of 8 k's:
 // 8 hi bf16 followed by 8 lo bf16 (32 bytes per group).
 // PI8 (Ozaki splitting): every element of a tensor is scaled by the tensor's
-// power of two 2^-e (|x| 2^-e < 1/2) and written as kI8Slices signed digits
-// d_1..d_s in [-64, 64], x = 2^e sum_t d_t 128^-t up to 2^e 128^-s / 2. A vK row
-// is stored as chunks of 64 k's, each chunk slice-major ([slice][64 bytes]); an
-// oK row as half-chunks of 32 k's ([slice][32 bytes]).
+// power of two 2^-e (|x| 2^-e <= 0.498), rounded to the integer N = rint(x 2^-e 256^S)
+// and written as S balanced base-256 digits d_1..d_S in [-128, 127], N = sum_t d_t
+// 256^(S-t), so x = 2^e sum_t d_t 256^-t up to 2^e 256^-S / 2. A vK row is stored as
+// chunks of 64 k's, each chunk slice-major ([slice][64 bytes]); an oK row as
+// half-chunks of 32 k's ([slice][32 bytes]).
 struct Operands {
     void *X = nullptr, *T = nullptr, *Rn = nullptr, *TM = nullptr;
     int64_t vK = 0, oK = 0;   // padded K extents (elements)
     int64_t rbv = 0, rbo = 0; // bytes per packed row of K extent vK / oK
-    int eX = 0, eT = 0, eR = 0;  // PI8: exponents of the (ia|bf), t2 and (ia|mj) scalings
+    int eX = 0, eT = 0, eR = 0;  // PI8: exponents of the (ia|bf), t2 and (ia|mj) scalings (eR == eX)
+    int slices = 0;              // PI8: digits per element
 };
 
 // Which source tensor a pack launch reads.
@@ -65,10 +69,10 @@ enum PackMode : int { PACK_X = 0, PACK_T = 1, PACK_RN = 2, PACK_TM = 3, PACK_S =
 
 hipError_t launch_pack(int mode, int prec, const double *src, void *dst, int o, int v, int64_t kpad,
                        hipStream_t s);
-// PI8 pack: the elements of mode's tensor times `scale` (a power of two) as kI8Slices digits,
-// in chunks of `gran` k's (64 for vK rows, 32 for oK rows)
+// PI8 pack: the elements of mode's tensor times `scale` (a power of two, |x| scale <= 0.498) as
+// `slices` digits, in chunks of `gran` k's (64 for vK rows, 32 for oK rows)
 hipError_t launch_pack_i8(int mode, const double *src, void *dst, int o, int v, int64_t kpad, int gran,
-                          double scale, hipStream_t s);
+                          double scale, int slices, hipStream_t s);
 // max |x| over n doubles into *out (device; *out must be zero before the call), as a bit pattern
 hipError_t launch_absmax(const double *x, int64_t n, unsigned long long *out, hipStream_t s);
 
@@ -85,11 +89,13 @@ struct WBuildArgs {
     // 1: an entry with i == j builds only H = the family-1 segments {0, 1, 3, 4, 5, 6} into W1
     // (W = H(a,b,c) + H(b,a,c) for i == j) and its family-2 work-groups do nothing
     int halve;
-    // PI8: W = scale_f * (f-term digit products) + scale_m * (m-term digit products); oK rows are
-    // read as half-chunks of 32 k's (ncho per row), two per 64-deep chunk; `zero` is >= 32 *
-    // kI8Slices bytes of zeros for the unused half of the last chunk
-    double scale_f, scale_m;
+    // PI8: W = scale * (digit products of the f- and the m-terms; the two integral blocks share one
+    // power-of-two scale and T2 serves both terms); oK rows are read as half-chunks of 32 k's (ncho per
+    // row), two per 64-deep chunk; `zero` is >= 32 * slices bytes of zeros for the unused half of the
+    // last chunk
+    double scale;
     const char *zero;
+    int slices;  // PI8 digits per element
 };
 
 // Tile configuration: 2x2 waves, each owning blk x blk MFMA 16x16 blocks.

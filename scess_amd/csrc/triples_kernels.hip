@@ -136,22 +136,22 @@ __global__ void pack_kernel(int mode, const double *__restrict__ src, void *__re
     }
 }
 
-// Ozaki splitting of one element: m = x * scale (|m| < 1/2, exact), then per slice
-// y = 128 m, d = rint(y), m = y - d (all exact in FP64); |d| <= 64.
+// Ozaki splitting of one element: m = x * scale (exact, |m| <= 0.498), N = rint(m 256^S) (|N| <
+// 0.4981 256^S fits S balanced base-256 digits), then from the least significant digit up
+// d = ((N + 128) mod 256) - 128 in [-128, 127] and N = (N - d) / 256, all exact in int64.
 __global__ void pack_i8_kernel(int mode, const double *__restrict__ src, int8_t *__restrict__ dst, int o_, int v_,
-                               int64_t kpad, int gran, double scale, int64_t nrows) {
+                               int64_t kpad, int gran, double scale, int S, int64_t nrows) {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= nrows * kpad)
         return;
     const int64_t row = gid / kpad, k = gid - row * kpad;
-    double m = pack_value(mode, src, o_, v_, row, k) * scale;
-    int8_t *d = dst + row * kpad * kI8Slices + (k / gran) * gran * kI8Slices + (k % gran);
-#pragma unroll
-    for (int t = 0; t < kI8Slices; t++) {
-        const double y = m * 128.0;
-        const double q = rint(y);
-        m = y - q;
-        d[t * gran] = (int8_t)(int)q;
+    const double m = pack_value(mode, src, o_, v_, row, k) * scale;
+    long long N = llrint(ldexp(m, 8 * S));
+    int8_t *d = dst + row * kpad * S + (k / gran) * gran * S + (k % gran);
+    for (int t = S - 1; t >= 0; t--) {
+        const long long dig = ((N + 128) & 255) - 128;
+        d[t * gran] = (int8_t)dig;
+        N = (N - dig) / 256;
     }
 }
 
@@ -167,15 +167,16 @@ static int64_t pack_rows(int mode, int64_t o, int64_t v) {
 }
 
 hipError_t launch_pack_i8(int mode, const double *src, void *dst, int o, int v, int64_t kpad, int gran,
-                          double scale, hipStream_t s) {
+                          double scale, int slices, hipStream_t s) {
     const int64_t nrows = pack_rows(mode, o, v);
-    if (nrows < 0 || mode == PACK_S || (gran != 32 && gran != 64) || kpad % gran)
+    if (nrows < 0 || mode == PACK_S || (gran != 32 && gran != 64) || kpad % gran || slices < kI8MinDigits ||
+        slices > kI8MaxDigits)
         return hipErrorInvalidValue;
     const int64_t blocks = (nrows * kpad + 255) / 256;
     if (blocks > 0x7fffffffLL)
         return hipErrorInvalidValue;
     hipLaunchKernelGGL(pack_i8_kernel, dim3((unsigned)blocks), dim3(256), 0, s, mode, src, (int8_t *)dst, o, v,
-                       kpad, gran, scale, nrows);
+                       kpad, gran, scale, slices, nrows);
     return hipGetLastError();
 }
 
@@ -715,23 +716,23 @@ __global__ __launch_bounds__(256) void wbuild_dma_kernel(WBuildArgs a) {
 }
 
 // ------------------------------------------------------ W build, int8 slices
-// Exact-product mode (PI8). Every operand element is S = kI8Slices signed digits (pack_i8_kernel);
-// the product of two elements is sum_{t,u} d_t d'_u 128^-(t+u) times the two tensor scales. The
-// leading pairs t + u <= S + 1 are summed exactly on v_mfma_i32_16x16x64_i8 into one int32
-// accumulator per g = t + u; at the end of the f-terms and of the m-terms (each with its own scale)
-// the groups are converted and added to FP64 accumulators. Dropped pairs and the digits past S
-// bound the product error by about (S + 1) 128^-S of max|A| max|B| (2^-46 for S = 7).
+// Exact-product mode (PI8). Every operand element is S signed 8-bit digits (pack_i8_kernel); the
+// product of two elements is sum_{t,u} d_t d'_u 256^-(t+u) times the two tensor scales. The leading
+// pairs t + u <= S + 1 (S (S + 1) / 2 of them) are summed exactly on v_mfma_i32_16x16x64_i8 into one
+// int32 accumulator per g = t + u over the whole K of the f- and the m-terms (the two integral blocks
+// share one power-of-two scale, and T2 serves both terms), and the groups are converted to FP64 once
+// at the end. The digits past S and the dropped pairs bound the product error by about
+// (S + 1) 256^-S of max|A| max|B|.
 //
 // Work-group: 2 x 2 waves on a 64 x 64 tile (each wave 2 x 2 MFMA 16 x 16 blocks), chunks of 64 k
 // (all S slices) streamed into LDS by global_load_lds_dwordx4 through two stages. A row of a chunk
-// is S x 64 bytes, slice-major; the four 16-byte vectors of a slice are XOR-permuted by (r >> 2) & 3,
-// which puts the 16 rows of an MFMA fragment read on distinct banks (row pitch 448 bytes for S = 7).
+// is S x 64 bytes, slice-major; the four 16-byte vectors of a slice are XOR-permuted by (r >> 2) & 3.
 // The m-terms (K = o, padded to 32) are read as half-chunks: the two halves of a chunk may come
 // from two segments, the missing half of an odd count from a zero buffer.
 typedef int i4v __attribute__((ext_vector_type(4)));
 
 template <int S>
-__global__ __launch_bounds__(256) void wbuild_i8_kernel(WBuildArgs a) {
+__global__ __launch_bounds__(256, 2) void wbuild_i8_kernel(WBuildArgs a) {
     constexpr int BM = 64;
     constexpr int RB = S * 64;     // bytes of one 64-deep chunk of a row
     constexpr int OPB = BM * RB;   // bytes of one operand chunk
@@ -777,19 +778,14 @@ __global__ __launch_bounds__(256) void wbuild_i8_kernel(WBuildArgs a) {
         vc[b] = c0 + wn * 32 + b * 16 < v;
     }
 
-    i4v acc[2][2][S];  // [x][y][g - 2]
-    double accd[2][2][4];
+    i4v acc[2][2][S];  // [x][y][g - 2]: f- and m-term digit products share one scale (see the flush)
 #pragma unroll
     for (int x = 0; x < 2; x++)
 #pragma unroll
-        for (int y = 0; y < 2; y++) {
+        for (int y = 0; y < 2; y++)
 #pragma unroll
             for (int g = 0; g < S; g++)
                 acc[x][y][g] = i4v{0, 0, 0, 0};
-#pragma unroll
-            for (int r = 0; r < 4; r++)
-                accd[x][y][r] = 0.0;
-        }
 
     // this lane's LDS units: tile row, slice and logical 16-byte vector of the slice
     int lrow[NIW], loff[NIW], lhalf[NIW], loffm[NIW];
@@ -852,50 +848,34 @@ __global__ __launch_bounds__(256) void wbuild_i8_kernel(WBuildArgs a) {
     auto frag = [&](const char *op, int rr, int t) {
         return *(const i4v *)(op + rr * RB + t * 64 + ((lq ^ ((rr >> 2) & 3)) * 16));
     };
+    // all S digits of the B fragments stay in registers; the A digits are loaded one at a time
     auto compute = [&](int st) {
         const char *As = lds[st];
         const char *Bs = As + OPB;
-        i4v af[2][S], bf[2][S];
+        i4v bf[2][S];
 #pragma unroll
         for (int b = 0; b < 2; b++)
 #pragma unroll
-            for (int t = 0; t < S; t++) {
-                af[b][t] = frag(As, wm * 32 + b * 16 + lr, t);
-                bf[b][t] = frag(Bs, wn * 32 + b * 16 + lr, t);
-            }
+            for (int u = 0; u < S; u++)
+                bf[b][u] = frag(Bs, wn * 32 + b * 16 + lr, u);
 #pragma unroll
-        for (int g = 2; g <= S + 1; g++)
+        for (int t = 1; t <= S; t++) {
+            i4v af[2];
 #pragma unroll
-            for (int t = 1; t <= S; t++)
-                if (g - t >= 1 && g - t <= S)
+            for (int b = 0; b < 2; b++)
+                af[b] = frag(As, wm * 32 + b * 16 + lr, t - 1);
 #pragma unroll
-                    for (int x = 0; x < 2; x++)
-                        if (vr[x])
+            for (int u = 1; t + u <= S + 1; u++)
 #pragma unroll
-                            for (int y = 0; y < 2; y++)
-                                if (vc[y])
-                                    acc[x][y][g - 2] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
-                                        af[x][t - 1], bf[y][g - t - 1], acc[x][y][g - 2], 0, 0, 0);
+                for (int x = 0; x < 2; x++)
+                    if (vr[x])
+#pragma unroll
+                        for (int y = 0; y < 2; y++)
+                            if (vc[y])
+                                acc[x][y][t + u - 2] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                                    af[x], bf[y][u - 1], acc[x][y][t + u - 2], 0, 0, 0);
+        }
     };
-    // accd += scale * sum_g acc_g 128^-g (smallest terms first); the int32 groups restart at 0
-    auto flush = [&](double scale) {
-#pragma unroll
-        for (int x = 0; x < 2; x++)
-#pragma unroll
-            for (int y = 0; y < 2; y++)
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    double sum = 0.0;
-#pragma unroll
-                    for (int g = S + 1; g >= 2; g--)
-                        sum += (double)acc[x][y][g - 2][r] * __builtin_ldexp(1.0, -7 * g);
-                    accd[x][y][r] = fma(scale, sum, accd[x][y][r]);
-#pragma unroll
-                    for (int g = 0; g < S; g++)
-                        acc[x][y][g][r] = 0;
-                }
-    };
-
     issue(0, 0);
     for (int c = 0; c < total; c++) {
         wait_vm<0>();
@@ -903,11 +883,10 @@ __global__ __launch_bounds__(256) void wbuild_i8_kernel(WBuildArgs a) {
         if (c + 1 < total)
             issue(c + 1, (c + 1) & 1);
         compute(c & 1);
-        if (c == nfc - 1)
-            flush(a.scale_f);
     }
-    flush(a.scale_m);
 
+    // W = scale * sum_g acc_g 256^-g (smallest terms first): every product of the f- and the m-terms
+    // carries the same tensor scales, 2^(eX + eT) = 2^(eR + eT)
     double *W = (double *)(fam1 ? a.W1 : a.W2) + (int64_t)bat * a.wstride;
     const int nbR = a.vR >> 3;
     const int64_t fbase = ((int64_t)(fx >> 3) * nbR) * nbR * 512 + (fx & 7) * 64;
@@ -919,9 +898,13 @@ __global__ __launch_bounds__(256) void wbuild_i8_kernel(WBuildArgs a) {
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int row = r0 + wm * 32 + x * 16 + 4 * lq + r;
+                double sum = 0.0;
+#pragma unroll
+                for (int g = S + 1; g >= 2; g--)
+                    sum += (double)acc[x][y][g - 2][r] * __builtin_ldexp(1.0, -8 * g);
                 if (row < v && col < v)
                     W[fbase + ((int64_t)(row >> 3) * nbR + (col >> 3)) * 512 + (row & 7) * 8 + (col & 7)] =
-                        accd[x][y][r];
+                        a.scale * sum;
             }
         }
 }
@@ -973,7 +956,12 @@ static hipError_t launch_wbuild_p(int blk, bool dma, const WBuildArgs &a, int nb
 hipError_t launch_wbuild(int prec, int blk, bool dma, const WBuildArgs &a, int nbatch, hipStream_t s) {
     if (prec == PI8) {  // one tile shape (64 x 64)
         const dim3 grid((unsigned)(2 * a.ntile1), (unsigned)nbatch);
-        hipLaunchKernelGGL((wbuild_i8_kernel<kI8Slices>), grid, dim3(256), 0, s, a);
+        switch (a.slices) {
+        case 5: hipLaunchKernelGGL((wbuild_i8_kernel<5>), grid, dim3(256), 0, s, a); break;
+        case 6: hipLaunchKernelGGL((wbuild_i8_kernel<6>), grid, dim3(256), 0, s, a); break;
+        case 7: hipLaunchKernelGGL((wbuild_i8_kernel<7>), grid, dim3(256), 0, s, a); break;
+        default: return hipErrorInvalidValue;
+        }
         return hipGetLastError();
     }
     if (prec == P64)

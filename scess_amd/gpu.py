@@ -99,6 +99,8 @@ def load(path: str = LIB_PATH):
     lib.scess_set_batch.restype = c.c_int
     lib.scess_t_set_inputs.argtypes = [c.c_void_p, c.POINTER(TIn), c.c_int]
     lib.scess_t_set_inputs.restype = c.c_int
+    lib.scess_t_set_int8_digits.argtypes = [c.c_void_p, c.c_int]
+    lib.scess_t_set_int8_digits.restype = c.c_int
     lib.scess_t_forward.argtypes = [c.c_void_p, c.c_int, _dp, c.POINTER(TStats)]
     lib.scess_t_forward.restype = c.c_int
     lib.scess_t_forward_ex.argtypes = [c.c_void_p, c.c_int, _dp, _dp, _dp, c.POINTER(TStats)]
@@ -189,6 +191,10 @@ class Context:
     def set_shard(self, shard: int, nshards: int):
         """Evaluate only shard `shard` of `nshards` equal ranges of occupied triples."""
         self.check(self.lib.scess_t_set_shard(self.h, int(shard), int(nshards)))
+
+    def set_int8_digits(self, digits: int):
+        """Digits per element of the "ozaki" mode (5, 6 or 7), used by the next set_triples_inputs."""
+        self.check(self.lib.scess_t_set_int8_digits(self.h, int(digits)))
 
     def set_triples_inputs(self, inp, precisions=("fp64",)):
         arrs = dict(e=_f64(inp.e), t1=_f64(inp.t1), t2=_f64(inp.t2), ovvv=_f64(inp.ovvv),

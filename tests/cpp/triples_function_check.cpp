@@ -5,7 +5,7 @@
 // quadratic in each entry, so the difference is exact up to rounding); dE/d<pq|rs> is zero outside the
 // three index patterns the functional reads; cancellation returns 0.0 and leaves no state to
 // differentiate; the FP32, BF16X3 and OZAKI forwards stay within 1e-5, 1e-4 and 1e-12 (relative) of
-// FP64. Prints one line per check; exit status 0 when all pass.
+// FP64, and the backward with the W recomputation in OZAKI stays within 1e-11 of the FP64 one. Prints one line per check; exit status 0 when all pass.
 #include "scess_cc.h"
 
 #include <array>
@@ -170,13 +170,20 @@ int main() {
             check(std::fabs(Eb - E) <= 1e-4 * std::fabs(E), "BF16X3 forward within 1e-4 (relative) of FP64", Eb, E);
             const double Ez = tm.forward(eri, e, o, t1, t2, nullptr, P::OZAKI);
             check(std::fabs(Ez - E) <= 1e-12 * std::fabs(E), "OZAKI forward within 1e-12 (relative) of FP64", Ez, E);
-            bool rejected = false;
-            try {
-                TriplesFunction bad(P::OZAKI);
-            } catch (const std::runtime_error &) {
-                rejected = true;
+            // W recomputed from int8 digit products in the backward: the gradients of the FP64 pair
+            TriplesFunction tz(P::OZAKI);
+            (void)tz.forward(eri, e, o, t1, t2);
+            const TriplesGradient gz = tz.backward(1.0);
+            double dmax = 0.0, gmax = 0.0;
+            for (int64_t n = 0; n < g1.t2.size(); n++) {
+                dmax = std::max(dmax, std::fabs(gz.t2.data()[n] - g1.t2.data()[n]));
+                gmax = std::max(gmax, std::fabs(g1.t2.data()[n]));
             }
-            check(rejected, "OZAKI W recomputation rejected", rejected ? 1.0 : 0.0, 1.0);
+            for (int64_t n = 0; n < g1.eri.size(); n++) {
+                dmax = std::max(dmax, std::fabs(gz.eri.data()[n] - g1.eri.data()[n]));
+                gmax = std::max(gmax, std::fabs(g1.eri.data()[n]));
+            }
+            check(dmax <= 1e-11 * gmax, "OZAKI W recomputation: backward within 1e-11 of the FP64 one", dmax, gmax);
         }
 
         // cancellation: forward returns 0.0 and leaves nothing to differentiate

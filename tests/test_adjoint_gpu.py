@@ -41,9 +41,12 @@ def ctx():
     c.close()
 
 
-@pytest.mark.parametrize("o,v,seed", [(2, 3, 1), (3, 5, 2), (4, 9, 3), (5, 17, 4), (3, 70, 5)])
+@pytest.mark.parametrize("o,v,seed", [(2, 3, 1), (3, 5, 2), (4, 9, 3), (5, 17, 4), (3, 70, 5), (6, 3, 6),
+                                      (34, 7, 7)])
 def test_backward_matches_oracle(ctx, o, v, seed):
-    """(3, 70): v past one 64-wide tile of the back-contraction GEMMs."""
+    """(3, 70): v past one 64-wide tile of the back-contraction GEMMs. (6, 3) and (34, 7): o > v, where
+    the (ooov) gradient (o^3 v entries) is the largest array the backward stages for download; (34, 7)
+    also has o past two 16-wide occupied chunks and N = o past 32 in the dRn and dTM back-contractions."""
     _say(f"case backward o={o} v={v}")
     inp = make_triples_inputs(o, v, seed)
     ctx.set_triples_inputs(inp, ("fp64",))
@@ -55,15 +58,14 @@ def test_backward_matches_oracle(ctx, o, v, seed):
         assert err < 1e-11, (k, err)
 
 
-@pytest.mark.skip(reason="the ozaki W recompute in the backward is switched off (scess_t_backward rejects it): "
-                         "its run at o=34, v=7 faulted the GPU and no cause was found by reading or by CPU "
-                         "emulation of the W-build and back-contraction addresses")
 @pytest.mark.parametrize("o,v,seed", [(4, 9, 31), (34, 7, 32)])
 def test_backward_with_ozaki_w_matches_oracle(ctx, o, v, seed):
-    """W recomputed from int8 digit products (~2^-46 product error) in the backward: the gradients keep
-    1e-11 relative to the oracle; o = 34 spans two 32-wide occupied half-chunks."""
+    """W recomputed from six 8-bit digit products per FP64 product (~2^-44 product error) in the
+    backward: the gradients keep 1e-11 relative to the oracle; o = 34 spans two 32-wide occupied
+    half-chunks and has o > v (the (ooov) gradient, o^3 v entries, is then the largest one)."""
     _say(f"case backward ozaki o={o} v={v}")
     inp = make_triples_inputs(o, v, seed)
+    ctx.set_int8_digits(6)
     ctx.set_triples_inputs(inp, ("fp64", "ozaki"))
     g = ctx.triples_backward("ozaki", dE=1.0)
     go = oracle.t_grad(inp)

@@ -46,21 +46,25 @@ def test_fp64_matches_oracle(ctx, o, v, seed):
     assert np.allclose(F, Fo, rtol=1e-11, atol=1e-13 * np.abs(Fo).max())
 
 
+@pytest.mark.parametrize("digits", [5, 6, 7])
 @pytest.mark.parametrize("o,v,seed", [(2, 3, 21), (5, 19, 22), (6, 33, 23), (8, 40, 24), (7, 70, 25),
                                       (3, 130, 26), (33, 9, 27), (40, 12, 28)])
-def test_ozaki_matches_oracle(ctx, o, v, seed):
-    """Int8-slice products are exact; W differs from FP64 by the dropped digit products
-    (~2^-46 of max|A| max|B| each), so E(T) keeps FP64's 1e-12 bar against the oracle. The sizes
-    cover v past one and two 64-deep chunks and o past one and two 32-wide half-chunks."""
-    _say(f"case ozaki o={o} v={v} seed={seed}")
+def test_ozaki_matches_oracle(ctx, o, v, seed, digits):
+    """Int8 digit products are exact; W differs from FP64 by the digits past S and the dropped digit
+    pairs (about (S+1) 2^-8S of max|A| max|B| each): E(T) within 1e-12 relative of the oracle for S = 6
+    and 7 (FP64's bar) and 1e-10 for S = 5, F_ijk within 1e-9 (S = 5) / 1e-12 of the largest F. The
+    sizes cover v past one and two 64-deep chunks and o past one and two 32-wide half-chunks."""
+    _say(f"case ozaki S={digits} o={o} v={v} seed={seed}")
     inp = make_triples_inputs(o, v, seed)
+    ctx.set_int8_digits(digits)
     ctx.set_triples_inputs(inp, ("fp64", "ozaki"))
     E, F, _ = ctx.triples_forward("ozaki", with_F=True)
     E64, F64, _ = ctx.triples_forward("fp64", with_F=True)
     Eo = oracle.t_abc(inp)
     _say(f"  E={E!r} Eo={Eo!r} rel={_rel(E, Eo):.3e} rel(fp64)={_rel(E64, Eo):.3e}")
-    assert _rel(E, Eo) < 1e-12, (E, Eo)
-    assert np.allclose(F, F64, rtol=1e-10, atol=1e-12 * np.abs(F64).max())
+    assert _rel(E, Eo) < (1e-10 if digits == 5 else 1e-12), (E, Eo)
+    assert np.abs(F - F64).max() <= (1e-9 if digits == 5 else 1e-12) * np.abs(F64).max()
+    ctx.set_int8_digits(6)
 
 
 def test_single_occupied_or_virtual_is_zero(ctx):
