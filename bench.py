@@ -4,9 +4,10 @@
 Workload: the synthetic (T) configuration at o=30, v=300 (generator in
 scess_amd/synthetic.py, seed 1), inputs resident in HBM before timing. A step
 is one full E(T) evaluation. Every precision mode in --modes (default FP64,
-FP32, split-BF16x3; "ozaki", the exact int8-digit mode, on request) is timed
-for --steps steps after --warmup untimed steps; |dE(T)| of each mode is
-measured against the FP64 GPU result of the same run.
+split-BF16x3 and "ozaki", the int8-digit mode with --int8-digits digits per
+element; FP32 on request) is timed for --steps steps after --warmup untimed
+steps; |dE(T)| of each mode is measured against the FP64 GPU result of the
+same run.
 
 `value` is the throughput (triples/s = o^3 v^3 / seconds per step) of the
 fastest mode whose |dE(T)| <= 1 microhartree; all modes are reported under
@@ -33,10 +34,14 @@ METRIC = "(T) triples/s & MFMA TFLOP/s, mixed vs FP64 GPU at |ΔE(T)|≤1µEh"
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2516.0 / 3.0, "ozaki": 5000.0}
 DTYPE = {"fp64": "f64", "fp32": "f32 (FP32 MFMA, FP32 accumulation; FP64 epilogue)",
          "bf16x3": "bf16x3 (split-BF16 MFMA, FP32 accumulation; FP64 epilogue)",
-         "ozaki": "i8x7 (FP64 operands as 7 int8 digits, exact int8 MFMA products, FP64 accumulation)"}
-# int8 multiply-adds per FP64 multiply-add in the ozaki mode: digit pairs t + u <= 8 of 7 x 7
-OZAKI_PAIRS = 28
+         "ozaki": "i8x{S} (FP64 operands as {S} signed 8-bit digits, exact int8 MFMA products of the {P} leading "
+                  "digit pairs, FP64 accumulation)"}
 BAR_EH = 1.0e-6
+
+
+def ozaki_pairs(digits: int) -> int:
+    """int8 multiply-adds per FP64 multiply-add in the ozaki mode: the digit pairs t + u <= S + 1."""
+    return digits * (digits + 1) // 2
 
 
 def log(*a):
@@ -44,7 +49,10 @@ def log(*a):
 
 
 def cpu_baseline(inp, seconds_target: float = 15.0):
-    """Time the oracle on a seeded sample of unique triples; scale to the full job."""
+    """Time the oracle (the abc-driven restatement of RCCSD_TriplesCorrection) on a seeded sample of
+    unique triples and scale to the full job, twice on the same sample: with GetW's two products in BLAS
+    dgemm (numpy's OpenBLAS, single-threaded per call, as the reference runs them with EIGEN_USE_BLAS;
+    this is the reported baseline) and with the oracle's own loop GEMMs."""
     import numpy as np
 
     import oracle
@@ -59,10 +67,12 @@ def cpu_baseline(inp, seconds_target: float = 15.0):
     nthreads = oracle.num_threads()
     ntrip = lib.oracle_num_unique_triples(inp.v)
     rng = np.random.default_rng(2024)
+    dgemm = oracle.blas_dgemm()
     t = time.perf_counter()
     prep = oracle.AbcPrepared(inp, lib=lib)  # serial repack of RCCSD_T.cpp:27-56, timed apart
     t_repack = time.perf_counter() - t
-    # calibrate on a small sample, then size the sample to about seconds_target
+    # calibrate on a small sample with BLAS, then size the sample to about seconds_target
+    lib.oracle_set_dgemm(dgemm)
     n0 = max(nthreads * 2, 16)
     items = np.sort(rng.choice(ntrip, size=min(n0, ntrip), replace=False))
     t = time.perf_counter()
@@ -70,22 +80,29 @@ def cpu_baseline(inp, seconds_target: float = 15.0):
     dt = time.perf_counter() - t
     n1 = int(min(ntrip, max(n0, n0 * seconds_target / max(dt, 1e-3))))
     items = np.sort(rng.choice(ntrip, size=n1, replace=False))
-    t = time.perf_counter()
-    prep.eval(items, nthreads)
-    dt = time.perf_counter() - t
+    timing = {}
+    for name, fn in (("blas", dgemm), ("loops", None)):
+        lib.oracle_set_dgemm(fn)
+        t = time.perf_counter()
+        prep.eval(items, nthreads)
+        timing[name] = time.perf_counter() - t
+    lib.oracle_set_dgemm(None)
     prep.close()
-    full = dt * ntrip / n1 + t_repack
     triples = float(inp.o) ** 3 * float(inp.v) ** 3
+    full = {k: dt * ntrip / n1 + t_repack for k, dt in timing.items()}
     return {
-        "value": triples / full,
+        "value": triples / full["blas"],
         "unit": "triples/s",
         "cores": nthreads,
         "kind": "port",
         "sample": (f"{n1} of {ntrip} unique virtual triples a>=b>=c ({100.0 * n1 / ntrip:.3f}%), seeded "
-                   f"uniform sample, {dt:.1f} s on {nthreads} threads ({arch} build), scaled linearly "
-                   f"(every triple costs the same, RCCSD_T.cpp:108) plus the one-time serial repack ({t_repack:.1f} s); "
-                   f"full job ~{full:.0f} s"),
-        "seconds_full_job_estimate": full,
+                   f"uniform sample, {timing['blas']:.1f} s on {nthreads} threads ({arch} build) with GetW in "
+                   f"BLAS dgemm (numpy's OpenBLAS, one thread per call), scaled linearly (every triple costs "
+                   f"the same, RCCSD_T.cpp:108) plus the one-time serial repack ({t_repack:.1f} s); full job "
+                   f"~{full['blas']:.0f} s"),
+        "seconds_full_job_estimate": full["blas"],
+        "loop_gemm_value": triples / full["loops"],
+        "loop_gemm_seconds_full_job_estimate": full["loops"],
     }
 
 
@@ -340,7 +357,9 @@ def main():
     ap.add_argument("--o", type=int, default=None, help="occupied orbitals (default 30; 20 --adjoint; 50 --ccsd)")
     ap.add_argument("--v", type=int, default=None, help="virtual orbitals (default 300; 200 --adjoint; 500 --ccsd)")
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--modes", default="fp64,fp32,bf16x3")
+    ap.add_argument("--modes", default="fp64,bf16x3,ozaki")
+    ap.add_argument("--int8-digits", type=int, default=5, choices=[5, 6, 7],
+                    help="digits per element of the ozaki mode")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--adjoint", action="store_true",
@@ -384,6 +403,7 @@ def main():
     ctx = gpu.Context(0)
     shard, nshards = (int(x) for x in args.shard.split("/"))
     ctx.set_shard(shard, nshards)
+    ctx.set_int8_digits(args.int8_digits)
     t0 = time.perf_counter()
     ctx.set_triples_inputs(inp, modes)
     log(f"[bench] inputs resident on the GPU in {time.perf_counter() - t0:.1f} s (PCIe upload + repack)")
@@ -434,18 +454,23 @@ def main():
     h = res[head]
 
     launch_ms = h["w_build_ms_per_step"] / max(h["w_build_launches_per_step"], 1)
-    # the ozaki W-build runs OZAKI_PAIRS int8 multiply-adds for every FP64 one: priced in int8 TOP/s
-    per_flop = OZAKI_PAIRS if head == "ozaki" else 1
+    # the ozaki W-build runs S(S+1)/2 int8 multiply-adds for every FP64 one: priced in int8 TOP/s
+    per_flop = ozaki_pairs(args.int8_digits) if head == "ozaki" else 1
     achieved = per_flop * h["flops_algorithmic"] / (h["w_build_ms_per_step"] * 1e-3) / 1e12
-    traffic = load_traffic(f"wbuild_{head}_o{o}_v{v}")
+    tag = f"wbuild_ozaki{args.int8_digits}_o{o}_v{v}" if head == "ozaki" else f"wbuild_{head}_o{o}_v{v}"
+    traffic = load_traffic(tag)
     roofline = {
         "bound": "mfma",
-        "kernel": "wbuild_i8_kernel<7>" if head == "ozaki" else f"wbuild_kernel<{head}>",
+        "kernel": f"wbuild_i8_kernel<{args.int8_digits}>" if head == "ozaki" else f"wbuild_kernel<{head}>",
         "achieved": achieved,
         "peak": PEAK_TFLOPS[head],
         "unit": "TOP/s (int8)" if head == "ozaki" else "TFLOP/s",
         "frac": achieved / PEAK_TFLOPS[head],
+        # HBM bytes per launch are not counted in this run: they come from a separate rocprofv3 --pmc run
+        # (FETCH_SIZE and WRITE_SIZE, scaled by a plain copy) recorded in the named profile
         "traffic": traffic.get("bytes_per_launch") if traffic else None,
+        "traffic_source": f"profiles/traffic_{tag}.md (separate rocprofv3 --pmc run of this kernel)"
+                          if traffic else None,
         "avg_launch_ms": launch_ms,
         "algorithmic_flops_per_launch": h["flops_algorithmic"] / max(h["w_build_launches_per_step"], 1),
         "executed_over_algorithmic": h["flops_executed"] / h["flops_algorithmic"],
@@ -469,12 +494,12 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": DTYPE[head],
+        "dtype": DTYPE[head].format(S=args.int8_digits, P=ozaki_pairs(args.int8_digits)),
         "data": "synthetic",
         "config": {
             "workload": f"closed-shell (T), o={o}, v={v}, synthetic generator seed {args.seed}"
                         + (f", occupied-triple shard {shard} of {nshards}" if nshards > 1 else ""),
-            "o": o, "v": v, "headline_mode": head,
+            "o": o, "v": v, "headline_mode": head, "int8_digits": args.int8_digits,
             "tflops_algorithmic": h["tflops_algorithmic"],
         },
         "roofline": roofline,

@@ -53,6 +53,8 @@ def _bind(lib):
     lib.oracle_t_grad_items.restype = None
     lib.oracle_t_grad_items.argtypes = [ctypes.c_int64, ctypes.c_int64, _dp, _dp, _dp, _dp, _dp, _dp,
                                         _i64p, ctypes.c_int64, _dp, _dp, _dp, _dp, _dp]
+    lib.oracle_set_dgemm.restype = None
+    lib.oracle_set_dgemm.argtypes = [ctypes.c_void_p]
     lib.oracle_num_unique_triples.restype = ctypes.c_int64
     lib.oracle_num_unique_triples.argtypes = [ctypes.c_int64]
     lib.oracle_num_unique_ijk.restype = ctypes.c_int64
@@ -89,6 +91,20 @@ def _p(a):
 def _args(inp):
     arrs = [_f(inp.e), _f(inp.t1), _f(inp.t2), _f(inp.ovvv), _f(inp.ovoo), _f(inp.ovov)]
     return arrs
+
+
+def blas_dgemm():
+    """Address of the CBLAS dgemm with 64-bit integers of numpy's bundled OpenBLAS (already loaded by
+    numpy), with its own threading set to one thread: for oracle_set_dgemm, which runs GetW's products in
+    BLAS as the reference does (EIGEN_USE_BLAS) while the oracle spreads the triples over threads."""
+    import glob
+    libs = glob.glob(os.path.join(os.path.dirname(np.__file__), "..", "numpy.libs", "*openblas*"))
+    if not libs:
+        raise RuntimeError("numpy's OpenBLAS not found")
+    blas = ctypes.CDLL(libs[0])
+    blas.scipy_openblas_set_num_threads64_.argtypes = [ctypes.c_int]
+    blas.scipy_openblas_set_num_threads64_(1)
+    return ctypes.cast(blas.scipy_cblas_dgemm64_, ctypes.c_void_p).value
 
 
 def num_threads() -> int:

@@ -88,9 +88,27 @@ typedef struct {
     atomic_llong next;
 } abc_job_t;
 
+/*
+ * Optional BLAS dgemm for GetW's two products: the CBLAS interface with 64-bit integers (as exported
+ * by an ILP64 OpenBLAS), set by oracle_set_dgemm. The reference runs these products through BLAS
+ * (Eigen with EIGEN_USE_BLAS, cmake/QuantumSCESS.cmake:79; RCCSD_T.cpp:64-67); NULL keeps the plain
+ * loops below. The caller makes the BLAS single-threaded: the triples are spread over threads here.
+ */
+typedef void (*cblas_dgemm64_t)(int layout, int transa, int transb, int64_t m, int64_t n, int64_t k, double alpha,
+                                const double *a, int64_t lda, const double *b, int64_t ldb, double beta, double *c,
+                                int64_t ldc);
+static cblas_dgemm64_t g_dgemm = NULL;
+enum { CBLAS_COL_MAJOR = 102, CBLAS_NO_TRANS = 111 };
+
+void oracle_set_dgemm(void *fn) { g_dgemm = (cblas_dgemm64_t)fn; }
+
 /* C(o x o^2) = A(o x v) B(v x o^2), all ColMajor (RCCSD_T.cpp:63-65) */
 static void gemm_w1(idx_t o, idx_t v, const double *restrict A, const double *restrict B, double *restrict C) {
     const idx_t o2 = o * o;
+    if (g_dgemm) {
+        g_dgemm(CBLAS_COL_MAJOR, CBLAS_NO_TRANS, CBLAS_NO_TRANS, o, o2, v, 1.0, A, o, B, v, 0.0, C, o);
+        return;
+    }
     idx_t n = 0;
     for (; n + 4 <= o2; n += 4) {
         double *c0 = C + o * n, *c1 = c0 + o, *c2 = c1 + o, *c3 = c2 + o;
@@ -125,6 +143,10 @@ static void gemm_w1(idx_t o, idx_t v, const double *restrict A, const double *re
 /* C(o^2 x o) -= A(o^2 x o) B(o x o), all ColMajor (RCCSD_T.cpp:66-67) */
 static void gemm_w2(idx_t o, const double *restrict A, const double *restrict B, double *restrict C) {
     const idx_t o2 = o * o;
+    if (g_dgemm) {
+        g_dgemm(CBLAS_COL_MAJOR, CBLAS_NO_TRANS, CBLAS_NO_TRANS, o2, o, o, -1.0, A, o2, B, o, 1.0, C, o2);
+        return;
+    }
     for (idx_t k = 0; k < o; k++) {
         double *c = C + o2 * k;
         for (idx_t m = 0; m < o; m++) {

@@ -198,3 +198,20 @@ def test_blas_ijk_restatement_matches_oracle():
                 continue
             Fb = t_ijk_blas(inp, int(i), int(j), int(k))
             assert abs(Fb - F[n]) <= 1e-12 * max(abs(F[n]), 1e-300), (o, v, (i, j, k), Fb, F[n])
+
+
+def test_abc_oracle_with_blas_dgemm_equals_loops():
+    """GetW's two products through BLAS dgemm (the reference's EIGEN_USE_BLAS path, used for the CPU
+    baseline) give the same E(T) as the oracle's loop GEMMs, 1e-14 relative."""
+    lib = oracle.load()
+    fn = oracle.blas_dgemm()
+    try:
+        for (o, v, seed) in ((3, 5, 1), (6, 13, 2), (7, 20, 3)):
+            inp = make_triples_inputs(o, v, seed)
+            lib.oracle_set_dgemm(None)
+            E0 = oracle.t_abc(inp)
+            lib.oracle_set_dgemm(fn)
+            E1 = oracle.t_abc(inp)
+            assert abs(E1 - E0) <= 1e-14 * abs(E0), (o, v, E0, E1)
+    finally:
+        lib.oracle_set_dgemm(None)
