@@ -473,6 +473,8 @@ def main():
                           if traffic else None,
         "avg_launch_ms": launch_ms,
         "algorithmic_flops_per_launch": h["flops_algorithmic"] / max(h["w_build_launches_per_step"], 1),
+        # the same launches priced as the FP64 contraction they replace (algorithmic FP64 flops per second)
+        "fp64_equivalent_tflops": h["flops_algorithmic"] / (h["w_build_ms_per_step"] * 1e-3) / 1e12,
         "executed_over_algorithmic": h["flops_executed"] / h["flops_algorithmic"],
     }
 

@@ -23,7 +23,6 @@ struct TriplesState {
     int o = 0, v = 0, vR = 0;
     bool have_prec[kNumPrec] = {false, false, false, false};
     Operands ops[kNumPrec];
-    char *zero = nullptr;  // zeros for the unused half-chunk of the PI8 W-build
     double *S = nullptr, *t1 = nullptr, *e = nullptr, *ev = nullptr;  // FP64 epilogue inputs
     std::vector<int> ijk_all;                          // 3 per unique occupied triple
     std::vector<int> work;                             // indices (into ijk_all) with i != k
@@ -104,7 +103,6 @@ void free_triples(TriplesState &t) {
         dfree(op.TM);
         op = Operands{};
     }
-    dfree(t.zero);
     dfree(t.S);
     dfree(t.t1);
     dfree(t.e);
@@ -349,8 +347,8 @@ int scess_t_set_inputs(scess_ctx *c, const scess_t_in *in, int prec_mask) {
             void **dst;
             int64_t rows, kpad;
             int gran, e;
-        } jobs[4] = {{PACK_X, ovvv, &op.X, o * v * v, op.vK, 64, op.eX},
-                     {PACK_T, t2, &op.T, o * o * v, op.vK, 64, op.eT},
+        } jobs[4] = {{PACK_X, ovvv, &op.X, o * v * v, op.vK, 32, op.eX},
+                     {PACK_T, t2, &op.T, o * o * v, op.vK, 32, op.eT},
                      {PACK_RN, ovoo, &op.Rn, o * o * v, op.oK, 32, op.eR},
                      {PACK_TM, t2, &op.TM, o * v * v, op.oK, 32, op.eT}};
         for (const Job &jb : jobs) {
@@ -363,13 +361,6 @@ int scess_t_set_inputs(scess_ctx *c, const scess_t_in *in, int prec_mask) {
             if (e != hipSuccess)
                 return hip_fail(c, e, "pack_i8 kernel");
         }
-        if (!t.zero) {
-            e = hipMalloc((void **)&t.zero, 1024);
-            if (e == hipSuccess)
-                e = hipMemsetAsync(t.zero, 0, 1024, c->stream);
-            if (e != hipSuccess)
-                return hip_fail(c, e, "zero buffer");
-        }
         return SCESS_OK;
     };
     if ((rc = pack(PACK_S, P64, r_ovov, (void **)&t.S, o * o * v, v))) {
@@ -381,9 +372,9 @@ int scess_t_set_inputs(scess_ctx *c, const scess_t_in *in, int prec_mask) {
         if (!(prec_mask & (1 << p)))
             continue;
         Operands &op = t.ops[p];
+        op.slices = p == PI8 ? c->int8_digits : 0;
         op.vK = round_up(v, kChunk[p]);
         op.oK = round_up(o, kChunkO[p]);
-        op.slices = p == PI8 ? c->int8_digits : 0;
         op.rbv = op.vK * kElemBytes[p] * (p == PI8 ? op.slices : 1);
         op.rbo = op.oK * kElemBytes[p] * (p == PI8 ? op.slices : 1);
         if (p == PI8) {
@@ -499,7 +490,6 @@ static WBuildArgs wbuild_args(const TriplesState &t, int prec, int blk, int64_t 
     wa.wstride = wstride;
     wa.halve = 1;
     wa.scale = std::ldexp(1.0, op.eX + op.eT);
-    wa.zero = t.zero;
     wa.slices = op.slices;
     return wa;
 }

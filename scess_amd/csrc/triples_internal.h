@@ -13,9 +13,8 @@ constexpr int kNumPrec = 4;
 
 // K-dimension chunk (elements) of each precision's W-build kernel; the K
 // extents of the packed operands are padded to a multiple of it with zeros.
-// PI8 pads vK to 64 (one int8 MFMA depth) and oK to 32 (two occupied half-chunks
-// share one 64-deep chunk).
-constexpr int kChunk[kNumPrec] = {16, 16, 32, 64};
+// PI8 pads vK and oK to 32 (the packed row chunk of the int8 W-build).
+constexpr int kChunk[kNumPrec] = {16, 16, 32, 32};
 constexpr int kChunkO[kNumPrec] = {16, 16, 32, 32};
 // Bytes of one packed element: FP64 8, FP32 4, BF16X3 4 (hi and lo bf16); PI8 one per digit (the
 // digit count is chosen per context, Operands::slices).
@@ -53,9 +52,8 @@ __device__ inline int xcd_remap(int w, int n) {
 // PI8 (Ozaki splitting): every element of a tensor is scaled by the tensor's
 // power of two 2^-e (|x| 2^-e <= 0.498), rounded to the integer N = rint(x 2^-e 256^S)
 // and written as S balanced base-256 digits d_1..d_S in [-128, 127], N = sum_t d_t
-// 256^(S-t), so x = 2^e sum_t d_t 256^-t up to 2^e 256^-S / 2. A vK row is stored as
-// chunks of 64 k's, each chunk slice-major ([slice][64 bytes]); an oK row as
-// half-chunks of 32 k's ([slice][32 bytes]).
+// 256^(S-t), so x = 2^e sum_t d_t 256^-t up to 2^e 256^-S / 2. Every row (vK or oK
+// long) is stored as chunks of 32 k's, each chunk slice-major ([slice][32 bytes]).
 struct Operands {
     void *X = nullptr, *T = nullptr, *Rn = nullptr, *TM = nullptr;
     int64_t vK = 0, oK = 0;   // padded K extents (elements)
@@ -70,7 +68,7 @@ enum PackMode : int { PACK_X = 0, PACK_T = 1, PACK_RN = 2, PACK_TM = 3, PACK_S =
 hipError_t launch_pack(int mode, int prec, const double *src, void *dst, int o, int v, int64_t kpad,
                        hipStream_t s);
 // PI8 pack: the elements of mode's tensor times `scale` (a power of two, |x| scale <= 0.498) as
-// `slices` digits, in chunks of `gran` k's (64 for vK rows, 32 for oK rows)
+// `slices` digits, in chunks of `gran` (32) k's
 hipError_t launch_pack_i8(int mode, const double *src, void *dst, int o, int v, int64_t kpad, int gran,
                           double scale, int slices, hipStream_t s);
 // max |x| over n doubles into *out (device; *out must be zero before the call), as a bit pattern
@@ -90,12 +88,9 @@ struct WBuildArgs {
     // (W = H(a,b,c) + H(b,a,c) for i == j) and its family-2 work-groups do nothing
     int halve;
     // PI8: W = scale * (digit products of the f- and the m-terms; the two integral blocks share one
-    // power-of-two scale and T2 serves both terms); oK rows are read as half-chunks of 32 k's (ncho per
-    // row), two per 64-deep chunk; `zero` is >= 32 * slices bytes of zeros for the unused half of the
-    // last chunk
+    // power-of-two scale and T2 serves both terms)
     double scale;
-    const char *zero;
-    int slices;  // PI8 digits per element
+    int slices;    // PI8 digits per element
 };
 
 // Tile configuration: 2x2 waves, each owning blk x blk MFMA 16x16 blocks.

@@ -715,35 +715,44 @@ __global__ __launch_bounds__(256) void wbuild_dma_kernel(WBuildArgs a) {
         }
 }
 
-// ------------------------------------------------------ W build, int8 slices
+// ------------------------------------------------------ W build, int8 digits
 // Exact-product mode (PI8). Every operand element is S signed 8-bit digits (pack_i8_kernel); the
 // product of two elements is sum_{t,u} d_t d'_u 256^-(t+u) times the two tensor scales. The leading
-// pairs t + u <= S + 1 (S (S + 1) / 2 of them) are summed exactly on v_mfma_i32_16x16x64_i8 into one
+// pairs t + u <= S + 1 (S (S + 1) / 2 of them) are summed exactly on v_mfma_i32_32x32x32_i8 into one
 // int32 accumulator per g = t + u over the whole K of the f- and the m-terms (the two integral blocks
 // share one power-of-two scale, and T2 serves both terms), and the groups are converted to FP64 once
 // at the end. The digits past S and the dropped pairs bound the product error by about
 // (S + 1) 256^-S of max|A| max|B|.
 //
-// Work-group: 2 x 2 waves on a 64 x 64 tile (each wave 2 x 2 MFMA 16 x 16 blocks), chunks of 64 k
-// (all S slices) streamed into LDS by global_load_lds_dwordx4 through two stages. A row of a chunk
-// is S x 64 bytes, slice-major; the four 16-byte vectors of a slice are XOR-permuted by (r >> 2) & 3.
-// The m-terms (K = o, padded to 32) are read as half-chunks: the two halves of a chunk may come
-// from two segments, the missing half of an odd count from a zero buffer.
+// Tiles (64 x 64, one fixed index, one triple) and segments as the floating-point W-build; each wave
+// owns one 32 x 32 block and the K loop runs in chunks of 32 k's, so one stage (both operands, S
+// digits) is 20 KB at S = 5: two stages and 147 registers admit three work-groups per CU. Every row of
+// every operand is packed in 32-k chunks ([slice][32 bytes]); the m-terms (K = o) are whole chunks.
+// A 32-k row chunk is 2 S sixteen-byte vectors (digit t, half h at vector 2 t + h); vector v of tile
+// row r sits at LDS slot r * 2S + (v ^ i8w_swz(r)), which puts the rows of every 8- and 16-lane
+// fragment read on distinct banks for S = 5, 6, 7.
 typedef int i4v __attribute__((ext_vector_type(4)));
+typedef int i16v __attribute__((ext_vector_type(16)));
 
-template <int S>
-__global__ __launch_bounds__(256, 2) void wbuild_i8_kernel(WBuildArgs a) {
+template <int S> __device__ inline int i8w_swz(int r) {
+    if constexpr (S == 6)
+        return ((r >> 1) ^ (r >> 2)) & 3;
+    else
+        return ((r >> 2) ^ (r >> 3)) & 1;
+}
+
+template <int S, int NST>
+__global__ __launch_bounds__(256, 2) void wbuild_i8w_kernel(WBuildArgs a) {
     constexpr int BM = 64;
-    constexpr int RB = S * 64;     // bytes of one 64-deep chunk of a row
+    constexpr int NV = 2 * S;      // 16-byte vectors of a row chunk
+    constexpr int RB = NV * 16;    // bytes of a row chunk (32 k's, S digits)
     constexpr int OPB = BM * RB;   // bytes of one operand chunk
-    constexpr int NIW = S;         // wave loads per operand per chunk (OPB / 4 KiB)
-    static_assert(OPB == NIW * 4096, "operand chunk must split into whole wave loads");
-    __shared__ __attribute__((aligned(1024))) char lds[2][2 * OPB];
+    static_assert(2 * OPB == S * 4096, "a stage must split into S wave loads per wave");
+    __shared__ __attribute__((aligned(1024))) char lds[NST][2 * OPB];
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = wave >> 1, wn = wave & 1;
-    const int lr = lane & 15, lq = lane >> 4;
 
     const int nt2 = a.nt * a.nt;
     const int gx = gridDim.x;
@@ -766,147 +775,112 @@ __global__ __launch_bounds__(256, 2) void wbuild_i8_kernel(WBuildArgs a) {
         return;  // the whole work-group: W2 of this entry is not used
     const int nfs = fam1 ? (deg ? 3 : 4) : 2;  // f segments (and as many m segments)
     const int nfc = nfs * a.nchv;              // f chunks
-    const int nhalf = nfs * a.ncho;            // m half-chunks
-    const int total = nfc + (nhalf + 1) / 2;
-    auto fseg = [&](int n) { return fam1 && deg ? halved_seg(n) : n; };
-    auto mseg = [&](int n) { return (fam1 ? 4 : 2) + n; };
-
-    bool vr[2], vc[2];
-#pragma unroll
-    for (int b = 0; b < 2; b++) {
-        vr[b] = r0 + wm * 32 + b * 16 < v;
-        vc[b] = c0 + wn * 32 + b * 16 < v;
-    }
-
-    i4v acc[2][2][S];  // [x][y][g - 2]: f- and m-term digit products share one scale (see the flush)
-#pragma unroll
-    for (int x = 0; x < 2; x++)
-#pragma unroll
-        for (int y = 0; y < 2; y++)
-#pragma unroll
-            for (int g = 0; g < S; g++)
-                acc[x][y][g] = i4v{0, 0, 0, 0};
-
-    // this lane's LDS units: tile row, slice and logical 16-byte vector of the slice
-    int lrow[NIW], loff[NIW], lhalf[NIW], loffm[NIW];
-#pragma unroll
-    for (int u = 0; u < NIW; u++) {
-        const int qq = (wave * NIW + u) * 64 + lane;
-        const int r = qq / (4 * S), p = qq - r * (4 * S);
-        const int t = p >> 2, lv = (p & 3) ^ ((r >> 2) & 3);
-        lrow[u] = r;
-        loff[u] = t * 64 + lv * 16;
-        lhalf[u] = lv >> 1;
-        loffm[u] = t * 32 + (lv & 1) * 16;
-    }
-
-    const char *pa[NIW], *pb[NIW];  // f chunks: this lane's addresses at chunk 0 of the segment
-    int cur_fs = -1;
-    auto issue = [&](int c, int st) {
-        char *base = lds[st];
+    const int total = nfc + nfs * a.ncho;
+    auto seg_of = [&](int c) {  // segment index (get_seg numbering) of chunk c
         if (c < nfc) {
-            const int fs = c / a.nchv, kc = c - fs * a.nchv;
-            if (fs != cur_fs) {
-                cur_fs = fs;
-                const Seg sg = get_seg(a, fam1, fseg(fs), i, j, k, fx);
+            const int n = c / a.nchv;
+            return fam1 && deg ? halved_seg(n) : n;
+        }
+        return (fam1 ? 4 : 2) + (c - nfc) / a.ncho;
+    };
+    auto kc_of = [&](int c) { return c < nfc ? c % a.nchv : (c - nfc) % a.ncho; };
+
+    const bool vr = r0 + wm * 32 < v, vc = c0 + wn * 32 < v;
+    i16v acc[S];
 #pragma unroll
-                for (int u = 0; u < NIW; u++) {
-                    const int gr = min(r0 + lrow[u], v - 1), gc = min(c0 + lrow[u], v - 1);
-                    pa[u] = sg.A + gr * sg.lda + loff[u];
-                    pb[u] = sg.B + gc * sg.ldb + loff[u];
-                }
-            }
-            const int64_t off = (int64_t)kc * RB;
+    for (int g = 0; g < S; g++)
+        acc[g] = i16v{};
+
+    // this lane's load units: operand (0 A, 1 B: uniform per wave load), tile row and byte offset of its
+    // logical vector in the row chunk
+    int lop[S], lrow[S], loff[S];
 #pragma unroll
-            for (int u = 0; u < NIW; u++) {
-                __builtin_amdgcn_global_load_lds((const void *)(pa[u] + off),
-                                                 (lds_void *)(base + (wave * NIW + u) * 1024), 16, 0, 0);
-                __builtin_amdgcn_global_load_lds((const void *)(pb[u] + off),
-                                                 (lds_void *)(base + OPB + (wave * NIW + u) * 1024), 16, 0, 0);
-            }
-        } else {
-            const int mc = c - nfc;
+    for (int u = 0; u < S; u++) {
+        const int qq = (wave * S + u) * 64 + lane;
+        const int op = qq / (BM * NV), slot = qq - op * (BM * NV);
+        const int r = slot / NV, vv = slot - r * NV;
+        lop[u] = op;
+        lrow[u] = r;
+        loff[u] = (vv ^ i8w_swz<S>(r)) * 16;
+    }
+    const char *ptr[S];  // this lane's source at chunk kc = 0 of the current segment
+    int cur_seg = -1;
+    auto issue = [&](int c, int st) {
+        const int sgi = seg_of(c);
+        if (sgi != cur_seg) {
+            cur_seg = sgi;
+            const Seg sg = get_seg(a, fam1, sgi, i, j, k, fx);
 #pragma unroll
-            for (int u = 0; u < NIW; u++) {
-                const int h = 2 * mc + lhalf[u];
-                const char *sa = a.zero + loffm[u], *sb = a.zero + loffm[u];
-                if (h < nhalf) {
-                    const int ms = h / a.ncho, kh = h - ms * a.ncho;
-                    const Seg sg = get_seg(a, fam1, mseg(ms), i, j, k, fx);
-                    const int gr = min(r0 + lrow[u], v - 1), gc = min(c0 + lrow[u], v - 1);
-                    sa = sg.A + gr * sg.lda + (int64_t)kh * (S * 32) + loffm[u];
-                    sb = sg.B + gc * sg.ldb + (int64_t)kh * (S * 32) + loffm[u];
-                }
-                __builtin_amdgcn_global_load_lds((const void *)sa, (lds_void *)(base + (wave * NIW + u) * 1024), 16,
-                                                 0, 0);
-                __builtin_amdgcn_global_load_lds((const void *)sb,
-                                                 (lds_void *)(base + OPB + (wave * NIW + u) * 1024), 16, 0, 0);
+            for (int u = 0; u < S; u++) {
+                const int gr = min((lop[u] ? c0 : r0) + lrow[u], v - 1);
+                ptr[u] = lop[u] ? sg.B + (int64_t)gr * sg.ldb + loff[u] : sg.A + (int64_t)gr * sg.lda + loff[u];
             }
         }
+        const int64_t off = (int64_t)kc_of(c) * RB;
+        char *base = lds[st];
+#pragma unroll
+        for (int u = 0; u < S; u++)
+            __builtin_amdgcn_global_load_lds((const void *)(ptr[u] + off),
+                                             (lds_void *)(base + (wave * S + u) * 1024), 16, 0, 0);
     };
-    // slice t, logical vector lv of tile row rr
-    auto frag = [&](const char *op, int rr, int t) {
-        return *(const i4v *)(op + rr * RB + t * 64 + ((lq ^ ((rr >> 2) & 3)) * 16));
+    const int rr = lane & 31, hh = lane >> 5;
+    auto frag = [&](const char *op, int row, int t) {
+        return *(const i4v *)(op + row * RB + (((2 * t + hh) ^ i8w_swz<S>(row)) * 16));
     };
-    // all S digits of the B fragments stay in registers; the A digits are loaded one at a time
+    // all S digits of the B fragment stay in registers; the A digits are loaded one at a time
     auto compute = [&](int st) {
+        if (!(vr && vc))
+            return;
         const char *As = lds[st];
         const char *Bs = As + OPB;
-        i4v bf[2][S];
+        i4v bf[S];
 #pragma unroll
-        for (int b = 0; b < 2; b++)
-#pragma unroll
-            for (int u = 0; u < S; u++)
-                bf[b][u] = frag(Bs, wn * 32 + b * 16 + lr, u);
+        for (int u = 0; u < S; u++)
+            bf[u] = frag(Bs, wn * 32 + rr, u);
 #pragma unroll
         for (int t = 1; t <= S; t++) {
-            i4v af[2];
-#pragma unroll
-            for (int b = 0; b < 2; b++)
-                af[b] = frag(As, wm * 32 + b * 16 + lr, t - 1);
+            const i4v af = frag(As, wm * 32 + rr, t - 1);
 #pragma unroll
             for (int u = 1; t + u <= S + 1; u++)
-#pragma unroll
-                for (int x = 0; x < 2; x++)
-                    if (vr[x])
-#pragma unroll
-                        for (int y = 0; y < 2; y++)
-                            if (vc[y])
-                                acc[x][y][t + u - 2] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
-                                    af[x], bf[y][u - 1], acc[x][y][t + u - 2], 0, 0, 0);
+                acc[t + u - 2] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af, bf[u - 1], acc[t + u - 2], 0, 0, 0);
         }
     };
-    issue(0, 0);
+
+#pragma unroll
+    for (int st = 0; st < NST - 1; st++)
+        if (st < total)
+            issue(st, st);
     for (int c = 0; c < total; c++) {
-        wait_vm<0>();
-        __syncthreads();  // every wave's chunk c is in LDS; the other stage is free again
-        if (c + 1 < total)
-            issue(c + 1, (c + 1) & 1);
-        compute(c & 1);
+        // this wave's loads of chunk c have landed once at most `ahead` later chunks are outstanding
+        const int ahead = min(total - 1 - c, NST - 2);
+        if (NST > 3 && ahead >= 2)
+            wait_vm<(NST > 3 ? 2 * S : 0)>();
+        else if (NST > 2 && ahead == 1)
+            wait_vm<(NST > 2 ? S : 0)>();
+        else
+            wait_vm<0>();
+        __syncthreads();  // every wave's chunk c is in LDS; stage (c - 1) % NST is free again
+        if (c + NST - 1 < total)
+            issue(c + NST - 1, (c + NST - 1) % NST);
+        compute(c % NST);
     }
 
-    // W = scale * sum_g acc_g 256^-g (smallest terms first): every product of the f- and the m-terms
-    // carries the same tensor scales, 2^(eX + eT) = 2^(eR + eT)
+    // W = scale * sum_g acc_g 256^-g; 32 x 32 result layout: register e of lane l holds row
+    // (e & 3) + 8 (e >> 2) + 4 (l >> 5), column l & 31
     double *W = (double *)(fam1 ? a.W1 : a.W2) + (int64_t)bat * a.wstride;
     const int nbR = a.vR >> 3;
     const int64_t fbase = ((int64_t)(fx >> 3) * nbR) * nbR * 512 + (fx & 7) * 64;
+    const int col = c0 + wn * 32 + rr;
 #pragma unroll
-    for (int x = 0; x < 2; x++)
+    for (int e = 0; e < 16; e++) {
+        const int row = r0 + wm * 32 + (e & 3) + 8 * (e >> 2) + 4 * hh;
+        double sum = 0.0;
 #pragma unroll
-        for (int y = 0; y < 2; y++) {
-            const int col = c0 + wn * 32 + y * 16 + lr;
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int row = r0 + wm * 32 + x * 16 + 4 * lq + r;
-                double sum = 0.0;
-#pragma unroll
-                for (int g = S + 1; g >= 2; g--)
-                    sum += (double)acc[x][y][g - 2][r] * __builtin_ldexp(1.0, -8 * g);
-                if (row < v && col < v)
-                    W[fbase + ((int64_t)(row >> 3) * nbR + (col >> 3)) * 512 + (row & 7) * 8 + (col & 7)] =
-                        a.scale * sum;
-            }
-        }
+        for (int g = S + 1; g >= 2; g--)
+            sum += (double)acc[g - 2][e] * __builtin_ldexp(1.0, -8 * g);
+        if (row < v && col < v)
+            W[fbase + ((int64_t)(row >> 3) * nbR + (col >> 3)) * 512 + (row & 7) * 8 + (col & 7)] = a.scale * sum;
+    }
 }
 
 // MFMA blocks per wave edge, from timing sweeps at v = 300: 2 (64x64 work-group tiles) for FP64
@@ -956,13 +930,20 @@ static hipError_t launch_wbuild_p(int blk, bool dma, const WBuildArgs &a, int nb
 hipError_t launch_wbuild(int prec, int blk, bool dma, const WBuildArgs &a, int nbatch, hipStream_t s) {
     if (prec == PI8) {  // one tile shape (64 x 64)
         const dim3 grid((unsigned)(2 * a.ntile1), (unsigned)nbatch);
-        switch (a.slices) {
-        case 5: hipLaunchKernelGGL((wbuild_i8_kernel<5>), grid, dim3(256), 0, s, a); break;
-        case 6: hipLaunchKernelGGL((wbuild_i8_kernel<6>), grid, dim3(256), 0, s, a); break;
-        case 7: hipLaunchKernelGGL((wbuild_i8_kernel<7>), grid, dim3(256), 0, s, a); break;
-        default: return hipErrorInvalidValue;
-        }
-        return hipGetLastError();
+        // two stages: three work-groups per CU beat deeper prefetch with two (sweep at o=30, v=300)
+        int nst = 2;
+        if (const char *e = std::getenv("SCESS_I8_STAGES"))  // tuning override: 2, 3 or 4
+            nst = std::atoi(e) >= 2 && std::atoi(e) <= 4 ? std::atoi(e) : nst;
+#define SCESS_I8W_CASE(S_, N_)                                                                   \
+    if (a.slices == S_ && nst == N_) {                                                           \
+        hipLaunchKernelGGL((wbuild_i8w_kernel<S_, N_>), grid, dim3(256), 0, s, a);               \
+        return hipGetLastError();                                                                \
+    }
+        SCESS_I8W_CASE(5, 2) SCESS_I8W_CASE(5, 3) SCESS_I8W_CASE(5, 4)
+        SCESS_I8W_CASE(6, 2) SCESS_I8W_CASE(6, 3) SCESS_I8W_CASE(6, 4)
+        SCESS_I8W_CASE(7, 2) SCESS_I8W_CASE(7, 3) SCESS_I8W_CASE(7, 4)
+#undef SCESS_I8W_CASE
+        return hipErrorInvalidValue;
     }
     if (prec == P64)
         return launch_wbuild_p<P64>(blk, dma, a, nbatch, s);
