@@ -1177,6 +1177,125 @@ __global__ __launch_bounds__(EPI_THREADS) void epilogue_kernel(EpiArgs a) {
     }
 }
 
+// The same epilogue on 256 threads (4 waves), so that its work-group fits the register, LDS and wave
+// slot of a W-build work-group: row-task rt (distinct block rt >> 6, a' = (rt >> 3) & 7, b' = rt & 7)
+// of an orbit's 384 is taken by thread rt and, for rt >= 256, by thread rt - 256. Each thread keeps W/D
+// of its rows in registers; Y goes through LDS (its own row included).
+constexpr int EPI2_THREADS = 256;
+
+template <typename AccT>
+__device__ inline void epilogue256_body(const EpiArgs &a, int grp, int bat, double *Yl, double *red) {
+    constexpr int EB = 8;
+    constexpr int PY = 9, PX = EB * PY + 1, PB = EB * PX;
+    const int tid = threadIdx.x;
+    const int *q = a.ijk + 3 * bat;
+    const int i = q[0], j = q[1], k = q[2];
+    const int v = a.v;
+    const int vR = a.vR, nbR = a.vR >> 3;
+    const int64_t vr = (int64_t)v * vR;
+    const double eocc = a.e[i] + a.e[j] + a.e[k];
+    const double *U = a.U + (int64_t)bat * 3 * vr;
+    const AccT *W1b = (const AccT *)a.W1 + (int64_t)bat * a.wstride;
+    const AccT *W2b = (const AccT *)(a.halve && i == j ? a.W1 : a.W2) + (int64_t)bat * a.wstride;
+    const int o0 = grp * EPI_RUN, o1 = min(o0 + EPI_RUN, a.norb);
+    const int nrow = tid < 384 - EPI2_THREADS ? 2 : 1;
+
+    double s = 0.0, c = 0.0;
+    for (int orb = o0; orb < o1; orb++) {
+        double WD[2][EB];
+        bool ok[2] = {false, false};
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            if (h >= nrow)
+                break;
+            const int rt = tid + EPI2_THREADS * h;
+            const OrbitRow rn = orbit_row_tab<PB, PX, PY>(a.orbtab, orb, rt);
+            if (!rn.own)
+                continue;
+            const int d = rt >> 6, ap = (rt >> 3) & 7, bp = rt & 7;
+            double Y[EB];
+            if (rn.ga < v && rn.gb < v) {
+                // W1 block (P,Q,R) row (a', b') and W2 block (Q,P,R) row (b', a'), 8 contiguous c'
+                const int P = rn.ga >> 3, Q = rn.gb >> 3, R = rn.gc0 >> 3;
+                const AccT *W1 = W1b + (((int64_t)P * nbR + Q) * nbR + R) * 512 + (rn.ga & 7) * 64 + (rn.gb & 7) * 8;
+                const AccT *W2 = W2b + (((int64_t)Q * nbR + P) * nbR + R) * 512 + (rn.gb & 7) * 64 + (rn.ga & 7) * 8;
+                const double u0 = U[(int64_t)rn.ga * vR + rn.gb];
+                const double *u1 = U + vr + (int64_t)rn.ga * vR + rn.gc0;
+                const double *u2 = U + 2 * vr + (int64_t)rn.gb * vR + rn.gc0;
+                const double *tk = a.t1 + (int64_t)k * vR + rn.gc0;
+                const double *evp = a.ev + rn.gc0;
+                const double ti = a.t1[(int64_t)i * vR + rn.ga], tj = a.t1[(int64_t)j * vR + rn.gb];
+                const double eab = eocc - a.ev[rn.ga] - a.ev[rn.gb];
+#pragma unroll
+                for (int cp = 0; cp < EB; cp++) {
+                    const double w = (double)W1[cp] + (double)W2[cp];
+                    Y[cp] = w + u0 * tk[cp] + u1[cp] * tj + u2[cp] * ti;
+                    WD[h][cp] = w * recip(eab - evp[cp]);
+                }
+                ok[h] = true;
+            } else {
+#pragma unroll
+                for (int cp = 0; cp < EB; cp++)
+                    Y[cp] = 0.0;
+            }
+#pragma unroll
+            for (int cp = 0; cp < EB; cp++)
+                Yl[d * PB + ap * PX + bp * PY + cp] = Y[cp];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            if (h >= nrow || !ok[h])
+                continue;
+            const int rt = tid + EPI2_THREADS * h;
+            const OrbitRow r = orbit_row_tab<PB, PX, PY>(a.orbtab, orb, rt);  // reloaded: fewer live registers
+            const int self = (rt >> 6) * PB + ((rt >> 3) & 7) * PX + (rt & 7) * PY;
+            // the row's (at most 8) terms in plain FP64, the rows compensated (Neumaier)
+            double rs = 0.0;
+#pragma unroll
+            for (int cp = 0; cp < EB; cp++) {
+                if (r.gc0 + cp < v) {  // Yl of masked columns is finite: padding is zero
+                    const double r3 = 4.0 * Yl[self + cp] + Yl[r.off[0] + cp * r.step[0]] +
+                                      Yl[r.off[1] + cp * r.step[1]] -
+                                      2.0 * (Yl[r.off[2] + cp * r.step[2]] + Yl[r.off[3] + cp * r.step[3]] +
+                                             Yl[r.off[4] + cp * r.step[4]]);
+                    rs += WD[h][cp] * r3;
+                }
+            }
+            nadd(s, c, rs);
+        }
+        __syncthreads();
+    }
+    // fixed-order reduction: xor butterfly in each wave, then the waves in order
+    wave_reduce(s, c);
+    const int wv = tid >> 6;
+    if ((tid & 63) == 0) {
+        red[2 * wv] = s;
+        red[2 * wv + 1] = c;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double ts = red[0], tc = red[1];
+        for (int w = 1; w < EPI2_THREADS / 64; w++) {
+            double ss, ee;
+            two_sum(ts, red[2 * w], ss, ee);
+            tc = tc + red[2 * w + 1] + ee;
+            ts = ss;
+        }
+        a.partials[(int64_t)bat * a.ngrp + grp] = ts + tc;
+    }
+}
+
+template <typename AccT>
+__global__ __launch_bounds__(EPI2_THREADS, 3) void epilogue256_kernel(EpiArgs a) {
+    constexpr int EB = 8, PY = 9, PX = EB * PY + 1, PB = EB * PX;
+    __shared__ double Yl[6 * PB];
+    __shared__ double red[2 * (EPI2_THREADS / 64)];
+    const int nx = gridDim.x;
+    const int lin = xcd_remap(blockIdx.y * nx + blockIdx.x, nx * gridDim.y);
+    epilogue256_body<AccT>(a, lin % nx, lin / nx, Yl, red);
+}
+
 int epilogue_groups(int norb) { return (norb + EPI_RUN - 1) / EPI_RUN; }
 
 bool orbit_table(const std::vector<int> &orbits, std::vector<uint64_t> &tab) {
@@ -1228,6 +1347,18 @@ bool orbit_table(const std::vector<int> &orbits, std::vector<uint64_t> &tab) {
 
 hipError_t launch_epilogue(int prec, const EpiArgs &a, int nbatch, hipStream_t s) {
     const dim3 grid((unsigned)a.ngrp, (unsigned)nbatch);
+    // 256 threads for the int8-digit mode, whose W-build leaves a work-group slot per CU that this
+    // epilogue fits; 384 for the others (timings at o=30, v=300). A/B override: 256 or 384.
+    bool narrow = prec == PI8;
+    if (const char *e = std::getenv("SCESS_EPI_THREADS"))
+        narrow = std::atoi(e) == 256 ? true : std::atoi(e) == 384 ? false : narrow;
+    if (narrow) {
+        if (prec == P64 || prec == PI8)
+            hipLaunchKernelGGL(epilogue256_kernel<double>, grid, dim3(EPI2_THREADS), 0, s, a);
+        else
+            hipLaunchKernelGGL(epilogue256_kernel<float>, grid, dim3(EPI2_THREADS), 0, s, a);
+        return hipGetLastError();
+    }
     if (prec == P64 || prec == PI8)  // both build W in FP64
         hipLaunchKernelGGL(epilogue_kernel<double>, grid, dim3(EPI_THREADS), 0, s, a);
     else
