@@ -39,6 +39,12 @@ DTYPE = {"fp64": "f64", "fp32": "f32 (FP32 MFMA, FP32 accumulation; FP64 epilogu
 BAR_EH = 1.0e-6
 
 
+# the W-build kernel of each mode (template arguments: precision, MFMA blocks per wave edge, stages;
+# int8: digits, fixed indices per work-group, stages)
+W_KERNEL = {"fp64": "wbuild_dma_kernel<0, 2, 2>", "fp32": "wbuild_dma_kernel<1, 2, 2>",
+            "bf16x3": "wbuild_dma_kernel<2, 4, 2>", "ozaki": "wbuild_i8f_kernel<{S}, {F}, 2>"}
+
+
 def ozaki_pairs(digits: int) -> int:
     """int8 multiply-adds per FP64 multiply-add in the ozaki mode: the digit pairs t + u <= S + 1."""
     return digits * (digits + 1) // 2
@@ -461,7 +467,7 @@ def main():
     traffic = load_traffic(tag)
     roofline = {
         "bound": "mfma",
-        "kernel": f"wbuild_i8_kernel<{args.int8_digits}>" if head == "ozaki" else f"wbuild_kernel<{head}>",
+        "kernel": W_KERNEL[head].format(S=args.int8_digits, F=2 if args.int8_digits == 5 else 1),
         "achieved": achieved,
         "peak": PEAK_TFLOPS[head],
         "unit": "TOP/s (int8)" if head == "ozaki" else "TFLOP/s",

@@ -672,7 +672,7 @@ int scess_t_forward_ex(scess_ctx *c, int prec, double *e_t, double *F_ijk, doubl
         if (he == hipSuccess)
             he = launch_vpieces(t.S, ea.ijk, (int)o, (int)v, (int)vR, U, nb, s1);
         if (he == hipSuccess)
-            he = launch_epilogue(prec, ea, nb, s1);
+            he = launch_epilogue(prec, ea, nb, s1, wa.slices);
         if (he == hipSuccess)
             he = launch_reduce_partials(pbase, t.ngrp, t.d_work_index + b0, t.F, nb, s1);
         if (he == hipSuccess)
@@ -833,7 +833,7 @@ int scess_t_forward_noalloc(scess_ctx *c, int prec, double *e_t) {
         if (he == hipSuccess)
             he = launch_vpieces(t.S, ea.ijk, (int)o, (int)v, (int)vR, U, 1, s);
         if (he == hipSuccess)
-            he = launch_epilogue(prec, ea, 1, s);
+            he = launch_epilogue(prec, ea, 1, s, wa.slices);
         if (he == hipSuccess)
             he = launch_reduce_partials(partials, t.ngrp, t.d_work_index + w, F, 1, s);
     }

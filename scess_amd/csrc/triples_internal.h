@@ -121,7 +121,10 @@ int epilogue_groups(int norb);
 // False when a block index does not fit the table's 10-bit fields (nb > kMaxOrbitBlocks).
 constexpr int kMaxOrbitBlocks = 1024;
 bool orbit_table(const std::vector<int> &orbits, std::vector<uint64_t> &tab);
-hipError_t launch_epilogue(int prec, const EpiArgs &a, int nbatch, hipStream_t s);
+// slices: int8 digits of a PI8 W-build (chooses the epilogue width that runs beside it)
+hipError_t launch_epilogue(int prec, const EpiArgs &a, int nbatch, hipStream_t s, int slices = 0);
+// fixed indices per work-group of the int8-digit W-build with `slices` digits
+int i8_fixed_per_group(int slices);
 // V pieces of every batch entry from S[p][q][x][y] = (px|qy) into U[nbatch][3][v][vR]
 hipError_t launch_vpieces(const double *S, const int *ijk, int o, int v, int vR, double *U, int nbatch, hipStream_t s);
 

@@ -724,164 +724,247 @@ __global__ __launch_bounds__(256) void wbuild_dma_kernel(WBuildArgs a) {
 // at the end. The digits past S and the dropped pairs bound the product error by about
 // (S + 1) 256^-S of max|A| max|B|.
 //
-// Tiles (64 x 64, one fixed index, one triple) and segments as the floating-point W-build; each wave
-// owns one 32 x 32 block and the K loop runs in chunks of 32 k's, so one stage (both operands, S
-// digits) is 20 KB at S = 5: two stages and 147 registers admit three work-groups per CU. Every row of
-// every operand is packed in 32-k chunks ([slice][32 bytes]); the m-terms (K = o) are whole chunks.
-// A 32-k row chunk is 2 S sixteen-byte vectors (digit t, half h at vector 2 t + h); vector v of tile
-// row r sits at LDS slot r * 2S + (v ^ i8w_swz(r)), which puts the rows of every 8- and 16-lane
-// fragment read on distinct banks for S = 5, 6, 7.
+// Tiles of 64 x 64 outputs of one triple and segments as the floating-point W-build; the K loop runs
+// in chunks of 32 k's. Every row of every operand is packed in 32-k chunks ([slice][32 bytes]); the
+// m-terms (K = o) are whole chunks. A 32-k row chunk is 2 S sixteen-byte vectors (digit t, half h at
+// vector 2 t + h); vector v of tile row r sits at LDS slot r * 2S + (v ^ i8_swz(r)), which puts the
+// rows of every 8- and 16-lane fragment read on distinct banks for S = 5, 6, 7.
+//
+// A work-group serves F fixed indices fx0 .. fx0 + F - 1. In every segment exactly one operand depends
+// on the fixed index (family 1: A in segments 0, 2, 6, 7 and B in 1, 3, 4, 5; family 2: B; see
+// get_seg), so a stage holds the other operand's 64-row tile once (tile 0, shared) and one 64-row tile
+// of the fixed-index operand per fx (tiles 1 .. F): F + 1 tiles for F outputs. F = 1: four waves, each
+// one 32 x 32 block (S int32 accumulators per lane; 139-189 registers, so the 256-thread epilogue fits
+// beside it); F = 2 (four waves) and F = 4 (eight waves, 512 threads): two waves per fixed index, each
+// owning 64 x 32 of its output (2 S accumulators), so that per chunk a wave reads 3 S fragments for
+// 2 S (S + 1) / 2 MFMAs instead of 2 S for S (S + 1) / 2.
 typedef int i4v __attribute__((ext_vector_type(4)));
 typedef int i16v __attribute__((ext_vector_type(16)));
 
-template <int S> __device__ inline int i8w_swz(int r) {
+template <int S> __device__ inline int i8_swz(int r) {
     if constexpr (S == 6)
         return ((r >> 1) ^ (r >> 2)) & 3;
     else
         return ((r >> 2) ^ (r >> 3)) & 1;
 }
 
-template <int S, int NST>
-__global__ __launch_bounds__(256, 2) void wbuild_i8w_kernel(WBuildArgs a) {
-    constexpr int BM = 64;
-    constexpr int NV = 2 * S;      // 16-byte vectors of a row chunk
-    constexpr int RB = NV * 16;    // bytes of a row chunk (32 k's, S digits)
-    constexpr int OPB = BM * RB;   // bytes of one operand chunk
-    static_assert(2 * OPB == S * 4096, "a stage must split into S wave loads per wave");
-    __shared__ __attribute__((aligned(1024))) char lds[NST][2 * OPB];
+__device__ inline bool seg_fx_on_a(bool fam1, int s) { return fam1 && (s == 0 || s == 2 || s == 6 || s == 7); }
+
+template <int F> constexpr int i8f_waves() { return F == 4 ? 8 : 4; }
+
+template <int S, int F, int NST>
+__global__ __launch_bounds__(F == 4 ? 512 : 256, (F == 1 || (F == 2 && S == 5)) ? 2 : 1) void wbuild_i8f_kernel(WBuildArgs a) {
+    static_assert(F == 1 || F == 2 || F == 4, "fixed indices per work-group");
+    static_assert(NST == 2 || NST == 3, "stages");
+    constexpr int NW = i8f_waves<F>();
+    constexpr int NR = F == 1 ? 1 : 2, NC = 1;               // 32 x 32 blocks per wave: rows x columns
+    constexpr int WPF = NW / F;                              // waves per fixed index
+    constexpr int NV = 2 * S;                                // 16-byte vectors of a 32-k row chunk
+    constexpr int RB = NV * 16;                              // bytes of a row chunk
+    constexpr int OPB = 64 * RB;                             // bytes of a 64-row tile chunk (NV KiB)
+    constexpr int NT = F + 1;                                // tiles per stage
+    constexpr int NL = NT * NV;                              // 1 KiB wave loads per stage
+    constexpr int NLW = (NL + NW - 1) / NW;                  // the most any wave issues
+    __shared__ __attribute__((aligned(1024))) char lds[NST][NT * OPB];
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int wm = wave >> 1, wn = wave & 1;
+    const int wf = wave / WPF, wp = wave - wf * WPF;  // this wave's fixed index (0 .. F-1), position
+    const int rb0 = F == 1 ? wp >> 1 : 0;
+    const int cb0 = F == 1 ? wp & 1 : wp;
 
     const int nt2 = a.nt * a.nt;
+    const int nfg = (a.v + F - 1) / F;  // fixed-index groups per family
     const int gx = gridDim.x;
     const int lin = xcd_remap(blockIdx.y * gx + blockIdx.x, gx * gridDim.y);
     const int bat = lin / gx;
     int bid = lin - bat * gx;
-    const bool fam1 = bid < a.ntile1;
+    const bool fam1 = bid < nfg * nt2;
     if (!fam1)
-        bid -= a.ntile1;
-    const int64_t fx = bid / nt2;
-    const int tt = bid - (int)fx * nt2;
+        bid -= nfg * nt2;
+    const int fg = bid / nt2;
+    const int tt = bid - fg * nt2;
     const int tr = tt / a.nt, tc = tt - tr * a.nt;
     const int *q = a.ijk + 3 * bat;
     const int64_t i = q[0], j = q[1], k = q[2];
     const int v = a.v;
-    const int r0 = tr * BM, c0 = tc * BM;
+    const int r0 = tr * 64, c0 = tc * 64;
+    const int64_t fx0 = (int64_t)fg * F;
 
     const bool deg = a.halve && i == j;
     if (deg && !fam1)
         return;  // the whole work-group: W2 of this entry is not used
     const int nfs = fam1 ? (deg ? 3 : 4) : 2;  // f segments (and as many m segments)
-    const int nfc = nfs * a.nchv;              // f chunks
-    const int total = nfc + nfs * a.ncho;
-    auto seg_of = [&](int c) {  // segment index (get_seg numbering) of chunk c
-        if (c < nfc) {
-            const int n = c / a.nchv;
-            return fam1 && deg ? halved_seg(n) : n;
-        }
-        return (fam1 ? 4 : 2) + (c - nfc) / a.ncho;
-    };
-    auto kc_of = [&](int c) { return c < nfc ? c % a.nchv : (c - nfc) % a.ncho; };
+    const int total = nfs * (a.nchv + a.ncho);
+    // segment (get_seg numbering) at position p of the chunk sequence: the f segments, then the m ones
+    auto seg_at = [&](int p) { return p < nfs ? (fam1 && deg ? halved_seg(p) : p) : (fam1 ? 4 : 2) + (p - nfs); };
 
-    const bool vr = r0 + wm * 32 < v, vc = c0 + wn * 32 < v;
-    i16v acc[S];
+    i16v acc[NR][NC][S];
 #pragma unroll
-    for (int g = 0; g < S; g++)
-        acc[g] = i16v{};
+    for (int x = 0; x < NR; x++)
+#pragma unroll
+        for (int y = 0; y < NC; y++)
+#pragma unroll
+            for (int g = 0; g < S; g++)
+                acc[x][y][g] = i16v{};
 
-    // this lane's load units: operand (0 A, 1 B: uniform per wave load), tile row and byte offset of its
-    // logical vector in the row chunk
-    int lop[S], lrow[S], loff[S];
-#pragma unroll
-    for (int u = 0; u < S; u++) {
-        const int qq = (wave * S + u) * 64 + lane;
-        const int op = qq / (BM * NV), slot = qq - op * (BM * NV);
-        const int r = slot / NV, vv = slot - r * NV;
-        lop[u] = op;
-        lrow[u] = r;
-        loff[u] = (vv ^ i8w_swz<S>(r)) * 16;
-    }
-    const char *ptr[S];  // this lane's source at chunk kc = 0 of the current segment
+    // load units: wave load L = wave + NW n of a stage fills LDS bytes [L KiB, L KiB + 1 KiB): tile L / NV,
+    // slot (L % NV) * 64 + lane = row * NV + physical vector. Each unit's source at chunk 0 of the current
+    // segment is formed when the segment changes; a chunk adds kc * RB.
+    const char *ptr[NLW];
     int cur_seg = -1;
-    auto issue = [&](int c, int st) {
-        const int sgi = seg_of(c);
-        if (sgi != cur_seg) {
-            cur_seg = sgi;
-            const Seg sg = get_seg(a, fam1, sgi, i, j, k, fx);
+    auto set_ptrs = [&](int sgi) {
+        const Seg s0 = get_seg(a, fam1, sgi, i, j, k, fx0);
+        const Seg s1 = get_seg(a, fam1, sgi, i, j, k, fx0 + 1);  // address arithmetic only: the step per fx
+        const bool fa = seg_fx_on_a(fam1, sgi);
+        const char *shb = fa ? s0.B : s0.A;
+        const int64_t shld = fa ? s0.ldb : s0.lda;
+        const int shr0 = fa ? c0 : r0;
+        const char *fxb = fa ? s0.A : s0.B;
+        const int64_t fxld = fa ? s0.lda : s0.ldb;
+        const int fxr0 = fa ? r0 : c0;
+        const int64_t dfx = fa ? s1.A - s0.A : s1.B - s0.B;
 #pragma unroll
-            for (int u = 0; u < S; u++) {
-                const int gr = min((lop[u] ? c0 : r0) + lrow[u], v - 1);
-                ptr[u] = lop[u] ? sg.B + (int64_t)gr * sg.ldb + loff[u] : sg.A + (int64_t)gr * sg.lda + loff[u];
+        for (int n = 0; n < NLW; n++) {
+            const int L = wave + NW * n;
+            if (L >= NL)
+                continue;
+            const int t = L / NV;
+            const int slot = (L - t * NV) * 64 + lane;
+            const int r = slot / NV, vv = slot - r * NV;
+            const int lo = (vv ^ i8_swz<S>(r)) * 16;
+            if (t == 0) {
+                ptr[n] = shb + (int64_t)min(shr0 + r, v - 1) * shld + lo;
+            } else {  // fixed indices past v load row fx = v - 1 (valid memory, outputs never stored)
+                const int64_t f = min(fx0 + t - 1, (int64_t)v - 1) - fx0;
+                ptr[n] = fxb + f * dfx + (int64_t)min(fxr0 + r, v - 1) * fxld + lo;
             }
         }
-        const int64_t off = (int64_t)kc_of(c) * RB;
+    };
+    // issue cursor (segment position sp, k-chunk kc) of the next chunk to load; bit st of famask: the
+    // chunk in stage st has its fixed-index operand on the A side (no divisions in the chunk loop)
+    int sp = 0, kc = 0;
+    unsigned famask = 0;
+    auto issue = [&](int st) {
+        const int sgi = seg_at(sp);
+        if (sgi != cur_seg) {
+            cur_seg = sgi;
+            set_ptrs(sgi);
+        }
+        famask = seg_fx_on_a(fam1, sgi) ? famask | (1u << st) : famask & ~(1u << st);
+        const int64_t off = (int64_t)kc * RB;
         char *base = lds[st];
 #pragma unroll
-        for (int u = 0; u < S; u++)
-            __builtin_amdgcn_global_load_lds((const void *)(ptr[u] + off),
-                                             (lds_void *)(base + (wave * S + u) * 1024), 16, 0, 0);
+        for (int n = 0; n < NLW; n++)
+            if (wave + NW * n < NL)
+                __builtin_amdgcn_global_load_lds((const void *)(ptr[n] + off),
+                                                 (lds_void *)(base + (wave + NW * n) * 1024), 16, 0, 0);
+        if (++kc == (sp < nfs ? a.nchv : a.ncho)) {
+            kc = 0;
+            ++sp;
+        }
     };
+    // per-lane byte offset of a fragment row in a tile: digit t of row r is logical vector 2 t + hh at
+    // physical vector (2 t + hh) ^ swz(r) = 4 (t >> 1) + ((2 (t & 1) + hh) ^ swz(r)) (swz < 4), so it sits at
+    // base[t & 1] + 64 (t >> 1) with two bases per row
     const int rr = lane & 31, hh = lane >> 5;
-    auto frag = [&](const char *op, int row, int t) {
-        return *(const i4v *)(op + row * RB + (((2 * t + hh) ^ i8w_swz<S>(row)) * 16));
-    };
-    // all S digits of the B fragment stay in registers; the A digits are loaded one at a time
-    auto compute = [&](int st) {
-        if (!(vr && vc))
-            return;
-        const char *As = lds[st];
-        const char *Bs = As + OPB;
-        i4v bf[S];
+    int aof[NR][2], bof[NC][2];
 #pragma unroll
-        for (int u = 0; u < S; u++)
-            bf[u] = frag(Bs, wn * 32 + rr, u);
+    for (int p = 0; p < 2; p++) {
+#pragma unroll
+        for (int x = 0; x < NR; x++) {
+            const int row = (rb0 + x) * 32 + rr;
+            aof[x][p] = row * RB + ((2 * p + hh) ^ i8_swz<S>(row)) * 16;
+        }
+#pragma unroll
+        for (int y = 0; y < NC; y++) {
+            const int row = (cb0 + y) * 32 + rr;
+            bof[y][p] = row * RB + ((2 * p + hh) ^ i8_swz<S>(row)) * 16;
+        }
+    }
+    const bool live = fx0 + wf < v;  // a wave whose fixed index is past v computes and stores nothing
+    // the S digits of the wave's NC B fragments stay in registers; the A digits are loaded one at a time
+    auto compute = [&](int st, bool fa) {
+        if (!live)
+            return;
+        const char *As = lds[st] + (fa ? 1 + wf : 0) * OPB;
+        const char *Bs = lds[st] + (fa ? 0 : 1 + wf) * OPB;
+        i4v bf[NC][S];
+#pragma unroll
+        for (int y = 0; y < NC; y++)
+#pragma unroll
+            for (int u = 0; u < S; u++)
+                bf[y][u] = *(const i4v *)(Bs + bof[y][u & 1] + 64 * (u >> 1));
 #pragma unroll
         for (int t = 1; t <= S; t++) {
-            const i4v af = frag(As, wm * 32 + rr, t - 1);
+            i4v af[NR];
+#pragma unroll
+            for (int x = 0; x < NR; x++)
+                af[x] = *(const i4v *)(As + aof[x][(t - 1) & 1] + 64 * ((t - 1) >> 1));
 #pragma unroll
             for (int u = 1; t + u <= S + 1; u++)
-                acc[t + u - 2] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af, bf[u - 1], acc[t + u - 2], 0, 0, 0);
+#pragma unroll
+                for (int x = 0; x < NR; x++)
+#pragma unroll
+                    for (int y = 0; y < NC; y++)
+                        acc[x][y][t + u - 2] =
+                            __builtin_amdgcn_mfma_i32_32x32x32_i8(af[x], bf[y][u - 1], acc[x][y][t + u - 2], 0, 0, 0);
         }
     };
 
+    const int nlw = (NL - wave + NW - 1) / NW;  // this wave's loads per stage: NLW or NLW - 1
 #pragma unroll
     for (int st = 0; st < NST - 1; st++)
         if (st < total)
-            issue(st, st);
+            issue(st);
+    int cst = 0;  // stage of chunk c
     for (int c = 0; c < total; c++) {
-        // this wave's loads of chunk c have landed once at most `ahead` later chunks are outstanding
-        const int ahead = min(total - 1 - c, NST - 2);
-        if (NST > 3 && ahead >= 2)
-            wait_vm<(NST > 3 ? 2 * S : 0)>();
-        else if (NST > 2 && ahead == 1)
-            wait_vm<(NST > 2 ? S : 0)>();
-        else
+        // this wave's loads of chunk c have landed once at most the next chunk's are outstanding
+        if (NST > 2 && c + 1 < total) {
+            if (nlw == NLW)
+                wait_vm<(NST > 2 ? NLW : 0)>();
+            else
+                wait_vm<(NST > 2 ? NLW - 1 : 0)>();
+        } else {
             wait_vm<0>();
+        }
         __syncthreads();  // every wave's chunk c is in LDS; stage (c - 1) % NST is free again
         if (c + NST - 1 < total)
-            issue(c + NST - 1, (c + NST - 1) % NST);
-        compute(c % NST);
+            issue(cst == 0 ? NST - 1 : cst - 1);  // stage (c + NST - 1) % NST
+        compute(cst, (famask >> cst) & 1);
+        cst = cst == NST - 1 ? 0 : cst + 1;
     }
+    if (!live)
+        return;
 
-    // W = scale * sum_g acc_g 256^-g; 32 x 32 result layout: register e of lane l holds row
+    // W = scale * sum_g acc_g 256^-g; register e of lane l of a 32 x 32 block holds row
     // (e & 3) + 8 (e >> 2) + 4 (l >> 5), column l & 31
     double *W = (double *)(fam1 ? a.W1 : a.W2) + (int64_t)bat * a.wstride;
     const int nbR = a.vR >> 3;
+    const int64_t fx = fx0 + wf;
     const int64_t fbase = ((int64_t)(fx >> 3) * nbR) * nbR * 512 + (fx & 7) * 64;
-    const int col = c0 + wn * 32 + rr;
 #pragma unroll
-    for (int e = 0; e < 16; e++) {
-        const int row = r0 + wm * 32 + (e & 3) + 8 * (e >> 2) + 4 * hh;
-        double sum = 0.0;
+    for (int x = 0; x < NR; x++)
 #pragma unroll
-        for (int g = S + 1; g >= 2; g--)
-            sum += (double)acc[g - 2][e] * __builtin_ldexp(1.0, -8 * g);
-        if (row < v && col < v)
-            W[fbase + ((int64_t)(row >> 3) * nbR + (col >> 3)) * 512 + (row & 7) * 8 + (col & 7)] = a.scale * sum;
-    }
+        for (int y = 0; y < NC; y++) {
+            const int col = c0 + (cb0 + y) * 32 + rr;
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                const int row = r0 + (rb0 + x) * 32 + (e & 3) + 8 * (e >> 2) + 4 * hh;
+                double sum = 0.0;
+#pragma unroll
+                for (int g = S + 1; g >= 2; g--)
+                    sum += (double)acc[x][y][g - 2][e] * __builtin_ldexp(1.0, -8 * g);
+                if (row < v && col < v)
+                    W[fbase + ((int64_t)(row >> 3) * nbR + (col >> 3)) * 512 + (row & 7) * 8 + (col & 7)] =
+                        a.scale * sum;
+            }
+        }
 }
+
+// Fixed indices per work-group of the int8-digit W-build, from timings at o=30, v=300 (W-build s per
+// step, F = 1 / 2 / 4): S = 5 5.49 / 5.25 / 5.48, S = 6 6.42 / 12.4 / 6.97, S = 7 8.62 / 14.2 / -.
+int i8_fixed_per_group(int slices) { return slices == 5 ? 2 : 1; }
 
 // MFMA blocks per wave edge, from timing sweeps at v = 300: 2 (64x64 work-group tiles) for FP64
 // and FP32; 4 for BF16x3, whose three MFMAs per product need the larger register tile to keep
@@ -928,21 +1011,23 @@ static hipError_t launch_wbuild_p(int blk, bool dma, const WBuildArgs &a, int nb
 }
 
 hipError_t launch_wbuild(int prec, int blk, bool dma, const WBuildArgs &a, int nbatch, hipStream_t s) {
-    if (prec == PI8) {  // one tile shape (64 x 64)
-        const dim3 grid((unsigned)(2 * a.ntile1), (unsigned)nbatch);
-        // two stages: three work-groups per CU beat deeper prefetch with two (sweep at o=30, v=300)
-        int nst = 2;
-        if (const char *e = std::getenv("SCESS_I8_STAGES"))  // tuning override: 2, 3 or 4
-            nst = std::atoi(e) >= 2 && std::atoi(e) <= 4 ? std::atoi(e) : nst;
-#define SCESS_I8W_CASE(S_, N_)                                                                   \
-    if (a.slices == S_ && nst == N_) {                                                           \
-        hipLaunchKernelGGL((wbuild_i8w_kernel<S_, N_>), grid, dim3(256), 0, s, a);               \
+    if (prec == PI8) {  // 64 x 64 tiles of F fixed indices per work-group
+        int nst = 2, f = i8_fixed_per_group(a.slices);
+        if (const char *e = std::getenv("SCESS_I8_STAGES"))  // tuning override: 2 or 3
+            nst = std::atoi(e) == 3 ? 3 : 2;
+        if (const char *e = std::getenv("SCESS_I8_FX"))  // tuning override: 1, 2 or 4
+            f = std::atoi(e);
+        const int nfg = (a.v + f - 1) / f;
+        const dim3 grid((unsigned)(2 * nfg * a.nt * a.nt), (unsigned)nbatch);
+#define SCESS_I8F_CASE(S_, F_, N_)                                                               \
+    if (a.slices == S_ && f == F_ && nst == N_) {                                                \
+        hipLaunchKernelGGL((wbuild_i8f_kernel<S_, F_, N_>), grid, dim3(64 * i8f_waves<F_>()), 0, s, a); \
         return hipGetLastError();                                                                \
     }
-        SCESS_I8W_CASE(5, 2) SCESS_I8W_CASE(5, 3) SCESS_I8W_CASE(5, 4)
-        SCESS_I8W_CASE(6, 2) SCESS_I8W_CASE(6, 3) SCESS_I8W_CASE(6, 4)
-        SCESS_I8W_CASE(7, 2) SCESS_I8W_CASE(7, 3) SCESS_I8W_CASE(7, 4)
-#undef SCESS_I8W_CASE
+        SCESS_I8F_CASE(5, 1, 2) SCESS_I8F_CASE(5, 2, 2) SCESS_I8F_CASE(5, 4, 2) SCESS_I8F_CASE(5, 4, 3)
+        SCESS_I8F_CASE(6, 1, 2) SCESS_I8F_CASE(6, 2, 2) SCESS_I8F_CASE(6, 4, 2)
+        SCESS_I8F_CASE(7, 1, 2) SCESS_I8F_CASE(7, 2, 2)
+#undef SCESS_I8F_CASE
         return hipErrorInvalidValue;
     }
     if (prec == P64)
@@ -1345,11 +1430,13 @@ bool orbit_table(const std::vector<int> &orbits, std::vector<uint64_t> &tab) {
     return true;
 }
 
-hipError_t launch_epilogue(int prec, const EpiArgs &a, int nbatch, hipStream_t s) {
+hipError_t launch_epilogue(int prec, const EpiArgs &a, int nbatch, hipStream_t s, int slices) {
     const dim3 grid((unsigned)a.ngrp, (unsigned)nbatch);
-    // 256 threads for the int8-digit mode, whose W-build leaves a work-group slot per CU that this
-    // epilogue fits; 384 for the others (timings at o=30, v=300). A/B override: 256 or 384.
-    bool narrow = prec == PI8;
+    // 256 threads beside an int8-digit W-build of one fixed index per work-group, which leaves a
+    // work-group slot per CU that this epilogue fits (5.31 s -> 5.17 s per step at o=30, v=300, S = 5);
+    // 384 beside the others (FP64 9.77 s against 9.91 s; S = 5 with two fixed indices 5.18 s against
+    // 5.25 s). A/B override: 256 or 384.
+    bool narrow = prec == PI8 && i8_fixed_per_group(slices) == 1;
     if (const char *e = std::getenv("SCESS_EPI_THREADS"))
         narrow = std::atoi(e) == 256 ? true : std::atoi(e) == 384 ? false : narrow;
     if (narrow) {
