@@ -157,6 +157,7 @@ def adjoint_bench(args):
     shard, nshards = (int(x) for x in args.shard.split("/"))
     ctx.set_shard(shard, nshards)
     wprec = args.adjoint_prec
+    ctx.set_int8_digits(args.int8_digits)
     ctx.set_triples_inputs(inp, tuple(dict.fromkeys(("fp64", wprec))))
     E, st = ctx.triples_forward(wprec)
     log(f"[adjoint] o={o} v={v} shard {shard}/{nshards} E(T)={E!r} forward ({wprec}) {st['seconds']:.3f} s")
@@ -371,7 +372,7 @@ def main():
     ap.add_argument("--adjoint", action="store_true",
                     help="time the (T) backward and check it by central differences (default o=20, v=200)")
     ap.add_argument("--fd-entries", type=int, default=64)
-    ap.add_argument("--adjoint-prec", default="fp64", choices=["fp64", "fp32", "bf16x3"],
+    ap.add_argument("--adjoint-prec", default="fp64", choices=["fp64", "fp32", "bf16x3", "ozaki"],
                     help="precision of the W recomputation in the backward (with --adjoint)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (scale for HBM byte counters)")

@@ -47,6 +47,38 @@ __global__ __launch_bounds__(256) void permute_kernel(PermArgs a) {
     a.out[lin] = a.beta == 0.0 ? x : a.beta * a.out[lin] + x;
 }
 
+// Permute into pre-split BF16x3 rows: element (row, k) of the permuted operand (a.odims, flattened
+// row * K + k) goes to row `row` of `out`, k padded to Kp (a multiple of 32) with zeros, as hi = bf16(x)
+// and lo = bf16(x - hi) (round to nearest even) in groups of 8 k: 8 hi then 8 lo (32 bytes per group).
+__device__ inline uint16_t bf16_rn(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__global__ __launch_bounds__(256) void permute_split_kernel(PermArgs a, int64_t K, int64_t Kp, uint16_t *out,
+                                                            int64_t nrows) {
+    const int64_t lin = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (lin >= nrows * Kp)
+        return;
+    const int64_t row = lin / Kp, k = lin - row * Kp;
+    double x = 0.0;
+    if (k < K) {
+        int64_t r = row * K + k, off = 0;
+        for (int d = a.nd - 1; d >= 0; d--) {
+            const int64_t q = r % a.odims[d];
+            r /= a.odims[d];
+            off += q * a.istride[d];
+        }
+        x = a.alpha * a.in[off];
+    }
+    const uint16_t hi = bf16_rn((float)x);
+    const uint16_t lo = bf16_rn((float)(x - (double)__uint_as_float((uint32_t)hi << 16)));
+    uint16_t *o = out + row * 2 * Kp + (k >> 3) * 16 + (k & 7);
+    o[0] = hi;
+    o[8] = lo;
+}
+
 // out = alpha * x + beta * y (elementwise)
 __global__ __launch_bounds__(256) void axpby_kernel(double *out, const double *x, const double *y, double alpha,
                                                     double beta, int64_t n) {
@@ -188,9 +220,9 @@ void Engine::axpby(double *out, const double *x, const double *y, double alpha, 
 }
 
 // out (labels lo, row-major dims) = beta*out + alpha*in[permuted], in has labels li
-void Engine::permute(const std::string &li, const double *in, const std::vector<int64_t> &idims,
-                     const std::string &lo, double *out, const std::vector<int64_t> &odims, double alpha,
-                     double beta) {
+static PermArgs perm_args(const std::string &li, const double *in, const std::vector<int64_t> &idims,
+                          const std::string &lo, double *out, const std::vector<int64_t> &odims, double alpha,
+                          double beta) {
     PermArgs a{};
     a.nd = (int)lo.size();
     if (a.nd > MAXD || (int)li.size() != a.nd)
@@ -214,10 +246,32 @@ void Engine::permute(const std::string &li, const double *in, const std::vector<
     a.out = out;
     a.alpha = alpha;
     a.beta = beta;
+    return a;
+}
+
+void Engine::permute(const std::string &li, const double *in, const std::vector<int64_t> &idims,
+                     const std::string &lo, double *out, const std::vector<int64_t> &odims, double alpha,
+                     double beta) {
+    const PermArgs a = perm_args(li, in, idims, lo, out, odims, alpha, beta);
     if (a.n == 0)
         return;
     hipLaunchKernelGGL(permute_kernel, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, stream, a);
     check(hipGetLastError(), "permute");
+}
+
+DTensor Engine::permute_split(const std::string &li, const double *in, const std::vector<int64_t> &idims,
+                              const std::string &lo, const std::vector<int64_t> &odims, int64_t K) {
+    const PermArgs a = perm_args(li, in, idims, lo, nullptr, odims, 1.0, 0.0);
+    if (K < 1 || a.n % K)
+        throw std::runtime_error("permute_split: K does not divide the operand");
+    const int64_t Kp = (K + 31) / 32 * 32, nrows = a.n / K;
+    DTensor out = alloc({nrows, Kp / 2});  // Kp bf16 pairs per row = Kp / 2 eight-byte slots
+    if (nrows > 0) {
+        hipLaunchKernelGGL(permute_split_kernel, dim3((unsigned)((nrows * Kp + 255) / 256)), dim3(256), 0, stream, a,
+                           K, Kp, (uint16_t *)out.p, nrows);
+        check(hipGetLastError(), "permute_split");
+    }
+    return out;
 }
 
 static void parse_spec(const std::string &spec, std::string &la, std::string &lb, std::string &lc) {
@@ -291,10 +345,20 @@ void Engine::einsum(const std::string &spec, const DTensor &A, const DTensor *B,
         return d;
     };
     const int64_t nb = prod(Lb), M = prod(Lm), N = prod(Ln), K = prod(Lk);
-    // operands in [batch][M][K] and [batch][N][K]
-    DTensor Ap = alloc({nb, M, K}), Bp = alloc({nb, N, K}), Cp = alloc({nb, M, N}, true);
-    permute(la, A.p, A.dims, Lb + Lm + Lk, Ap.p, dims_of(Lb + Lm + Lk), 1.0, 0.0);
-    permute(lb, B->p, B->dims, Lb + Ln + Lk, Bp.p, dims_of(Lb + Ln + Lk), 1.0, 0.0);
+    // operands in [batch][M][K] and [batch][N][K]: FP64, or (BF16x3 products) pre-split rows of
+    // Ks = Kp / 2 eight-byte slots, written by the permute itself
+    const bool split = prec == PB3 && M > 0 && N > 0 && K > 0;
+    const int64_t Ks = split ? (K + 31) / 32 * 16 : K;
+    DTensor Ap, Bp, Cp = alloc({nb, M, N}, true);
+    if (split) {
+        Ap = permute_split(la, A.p, A.dims, Lb + Lm + Lk, dims_of(Lb + Lm + Lk), K);
+        Bp = permute_split(lb, B->p, B->dims, Lb + Ln + Lk, dims_of(Lb + Ln + Lk), K);
+    } else {
+        Ap = alloc({nb, M, K});
+        Bp = alloc({nb, N, K});
+        permute(la, A.p, A.dims, Lb + Lm + Lk, Ap.p, dims_of(Lb + Lm + Lk), 1.0, 0.0);
+        permute(lb, B->p, B->dims, Lb + Ln + Lk, Bp.p, dims_of(Lb + Ln + Lk), 1.0, 0.0);
+    }
     if (M > 0 && N > 0 && K > 0) {
         GemmArgs g{};
         g.nseg = 1;
@@ -303,14 +367,14 @@ void Engine::einsum(const std::string &spec, const DTensor &A, const DTensor *B,
         g.alpha = 1.0;
         g.splits = 1;
         g.seg[0].K1 = 1;
-        g.seg[0].K2 = (int)K;
-        g.seg[0].A = {Ap.p, K, 0, 1, M * K};
-        g.seg[0].B = {Bp.p, K, 0, 1, N * K};
+        g.seg[0].K2 = (int)Ks;
+        g.seg[0].A = {Ap.p, Ks, 0, 1, M * Ks};
+        g.seg[0].B = {Bp.p, Ks, 0, 1, N * Ks};
         g.C = Cp.p;
         g.sCm = N;
         g.sCn = 1;
         g.sCb = M * N;
-        g.prec = prec;
+        g.prec = split ? PB3S : prec;
         if (M > 0x7fffffff || N > 0x7fffffff || K > 0x7fffffff || nb > 65535)
             throw std::runtime_error("einsum " + spec + ": extent too large");
         // few output tiles over a long K (e.g. Foo, the T1 terms): split K so that ~512 work-groups
@@ -379,6 +443,15 @@ void CcsdState::ladder_from_factor(Engine &E, const DTensor &tau, DTensor &R) {
         TAB /= 2;
     const int64_t nP = naux;
     DTensor I = E.alloc({TAB, v, TAB, v});  // I[a][c][b][d] of one tile
+    // BF16x3 products: the factor (rows (a, c), K = P), tau (rows (i, j, c), K = d) and each tile of I
+    // (rows (a, c, b), K = d) in pre-split rows of Ps / Vs eight-byte slots
+    const bool split = E.prec == PB3;
+    const int64_t Ps = (nP + 31) / 32 * 16, Vs = (v + 31) / 32 * 16;
+    DTensor Ls, taus;
+    if (split) {
+        Ls = E.permute_split("acP", Lvv.p, {v, v, nP}, "acP", {v, v, nP}, nP);
+        taus = E.permute_split("ijcd", tau.p, {o, o, v, v}, "ijcd", {o, o, v, v}, v);
+    }
     for (int64_t a0 = 0; a0 < v; a0 += TAB) {
         const int64_t ta = std::min(TAB, v - a0);
         for (int64_t b0 = 0; b0 < v; b0 += TAB) {
@@ -391,12 +464,18 @@ void CcsdState::ladder_from_factor(Engine &E, const DTensor &tau, DTensor &R) {
             g.alpha = 1.0;
             g.splits = 1;
             g.seg[0].K1 = 1;
-            g.seg[0].K2 = (int)nP;
-            g.seg[0].A = {Lvv.p + a0 * v * nP, nP, 0, 1, 0};  // rows (a, c), K = P contiguous
-            g.seg[0].B = {Lvv.p + b0 * v * nP, nP, 0, 1, 0};
+            if (split) {
+                g.seg[0].K2 = (int)Ps;
+                g.seg[0].A = {Ls.p + a0 * v * Ps, Ps, 0, 1, 0};
+                g.seg[0].B = {Ls.p + b0 * v * Ps, Ps, 0, 1, 0};
+            } else {
+                g.seg[0].K2 = (int)nP;
+                g.seg[0].A = {Lvv.p + a0 * v * nP, nP, 0, 1, 0};  // rows (a, c), K = P contiguous
+                g.seg[0].B = {Lvv.p + b0 * v * nP, nP, 0, 1, 0};
+            }
             check(hipMemsetAsync(I.p, 0, sizeof(double) * (size_t)I.size(), E.stream), "memset I");
             g.C = I.p;
-            g.prec = E.prec;
+            g.prec = split ? PB3S : E.prec;
             g.sCm = tb * v;
             g.sCn = 1;
             g.sCb = 0;
@@ -409,10 +488,18 @@ void CcsdState::ladder_from_factor(Engine &E, const DTensor &tau, DTensor &R) {
             h.alpha = 1.0;
             h.splits = 1;
             h.seg[0].K1 = (int)v;
-            h.seg[0].K2 = (int)v;
-            h.seg[0].A = {I.p, v, tb * v, 1, v * tb * v};
-            h.seg[0].B = {tau.p, v * v, v, 1, 0};
-            h.prec = E.prec;
+            DTensor Is;
+            if (split) {  // I of this tile in pre-split rows (a, c, b), K = d
+                Is = E.permute_split("acbd", I.p, {ta, v, tb, v}, "acbd", {ta, v, tb, v}, v);
+                h.seg[0].K2 = (int)Vs;
+                h.seg[0].A = {Is.p, Vs, tb * Vs, 1, v * tb * Vs};
+                h.seg[0].B = {taus.p, v * Vs, Vs, 1, 0};
+            } else {
+                h.seg[0].K2 = (int)v;
+                h.seg[0].A = {I.p, v, tb * v, 1, v * tb * v};
+                h.seg[0].B = {tau.p, v * v, v, 1, 0};
+            }
+            h.prec = split ? PB3S : E.prec;
             h.C = R.p + a0 * v + b0;
             h.sCb = v;
             h.sCm = 1;

@@ -37,6 +37,10 @@ struct Engine {
     void axpby(double *out, const double *x, const double *y, double alpha, double beta, int64_t n);
     void permute(const std::string &li, const double *in, const std::vector<int64_t> &idims, const std::string &lo,
                  double *out, const std::vector<int64_t> &odims, double alpha, double beta);
+    // the permutation of `in` (labels li -> lo, output dims odims) written as pre-split BF16x3 rows of its
+    // last K elements (rows padded to a multiple of 32 k); returned as rows x Kp/2 eight-byte slots
+    DTensor permute_split(const std::string &li, const double *in, const std::vector<int64_t> &idims,
+                          const std::string &lo, const std::vector<int64_t> &odims, int64_t K);
     // C = beta C + alpha einsum(spec, A, B); B == nullptr for a pure permutation
     void einsum(const std::string &spec, const DTensor &A, const DTensor *B, DTensor &C, double alpha, double beta);
     double dot(const double *x, const double *y, int64_t n);  // compensated, deterministic

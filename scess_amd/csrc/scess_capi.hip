@@ -887,11 +887,19 @@ int scess_t_backward(scess_ctx *c, int prec, double dE, scess_t_grad *grad) {
             (void)hipFree(p);
         bufs.clear();
     };
-    // K splits of dT (v x v outputs, K = v^2) and dRn (v x o outputs, K = v^2): enough work-groups
-    // (about 1024) that the long K loops overlap their memory latency across the chip
+    // K splits of dT (v x v outputs, K = v^2) and dRn (v x o outputs, K = v^2): about 1024 work-groups per
+    // GEMM so that the long K loops overlap their memory latency across the chip (the terms of a triple
+    // then run together, up to six GEMMs per launch). SCESS_ADJ_SPLITS = PT,PR overrides (tuning).
     const int64_t tilesT = ((v + 63) / 64) * ((v + 63) / 64), tilesR = ((v + 63) / 64) * ((o + 63) / 64);
-    const int PT = (int)std::min<int64_t>(64, std::max<int64_t>(16, 1024 / tilesT));
-    const int PR = (int)std::min<int64_t>(128, std::max<int64_t>(32, 1024 / tilesR));
+    int PT = (int)std::min<int64_t>(64, std::max<int64_t>(16, 1024 / tilesT));
+    int PR = (int)std::min<int64_t>(128, std::max<int64_t>(32, 1024 / tilesR));
+    if (const char *e = std::getenv("SCESS_ADJ_SPLITS")) {
+        int a = 0, b = 0;
+        if (std::sscanf(e, "%d,%d", &a, &b) == 2 && a >= 1 && a <= 256 && b >= 1 && b <= 256) {
+            PT = a;
+            PR = b;
+        }
+    }
     double *dX, *dT, *dRn, *dTM, *dS, *dt1, *GW, *GV, *U, *dU, *dtv, *scr, *out;
     void *W1, *W2;
     // W of up to WB triples is built by one launch (2 buffers of WB x vR^3 within 4 GB)
@@ -899,7 +907,7 @@ int scess_t_backward(scess_ctx *c, int prec, double dE, scess_t_grad *grad) {
     hipError_t he = hipSuccess;
     const size_t nX = (size_t)(o * v * v * vK), nT = (size_t)(o * o * v * vK), nR = (size_t)(o * o * v * oK),
                  nTM = (size_t)(o * v * v * oK), nS = (size_t)(o * o * v * v), nt1 = (size_t)(o * vR),
-                 nG = (size_t)wstride, nscr = (size_t)std::max<int64_t>(PT * v * v, PR * v * o);
+                 nG = (size_t)wstride, nscr = (size_t)kMaxGemmGroup * std::max<int64_t>(PT * v * v, PR * v * o);
     // staging of each unpacked gradient before its download: the largest of ovvv (o v^3), ovoo (o^3 v),
     // ovov and t2 (o^2 v^2) and t1 (o v)
     const size_t nout = std::max<size_t>({(size_t)(o * v * v * v), (size_t)(o * v * o * o), (size_t)(o * o * v * v),
@@ -972,6 +980,8 @@ int scess_t_backward(scess_ctx *c, int prec, double dE, scess_t_grad *grad) {
         ea.W2 = occ[0] == occ[1] ? ea.W1 : (const char *)W2 + (size_t)slot * wstride * accb;
         if (he == hipSuccess)
             he = launch_vpieces(t.S, dijk, (int)o, (int)v, (int)vR, U, 1, c->stream);
+        // G_W entries with an index in [v, vR) are zero: the buffer starts zeroed and the G kernel writes
+        // exact zeros past v (the back-contraction GEMMs read them as K padding)
         if (he == hipSuccess)
             he = launch_adjoint_g(prec, ea, dE * 2.0 / deg, GW, GV, c->stream);
         // With occ[0] == occ[1] (every triple with two equal indices, as evaluated) G is symmetric in its
@@ -982,6 +992,11 @@ int scess_t_backward(scess_ctx *c, int prec, double dE, scess_t_grad *grad) {
         // dX[p][x][y][f] += sum_z G(x,y,z) T[r][q][z][f]: the two permutation terms with the same
         // position for x (PERM rows 2m, 2m+1) write the same dX[p], so they run as one GEMM with
         // two K segments (twin: pair 1 equals pair 0, and the two segments of pair 2 are equal)
+        // The GEMMs of each kind write disjoint slices (dX[p] for the three values of p, dT[r][q] and
+        // dRn[p][q] for six distinct pairs, dTM[r] for three values), so each kind runs as one group, and
+        // every output element still receives this triple's contribution once, in triple order.
+        GemmGroup grp{};
+        grp.n = 0;
         for (int pr = 0; pr < 3 && he == hipSuccess; pr++) {
             if (twin && pr == 1)
                 continue;
@@ -999,15 +1014,20 @@ int scess_t_backward(scess_ctx *c, int prec, double dE, scess_t_grad *grad) {
                 GemmSeg &sg = g.seg[h];
                 sg.K1 = 1;
                 sg.K2 = (int)v;
-                sg.A = {GW, sx == 1 ? sx : sy, 0, sz, sx == 1 ? sy : sx};  // rows x batch y, or rows y batch x
-                sg.B = {T64 + (r * o + q) * v * vK, 1, 0, vK, 0};
+                // rows x batch y, or rows y batch x
+                sg.A = {GW, sx == 1 ? sx : sy, 0, sz, sx == 1 ? sy : sx, vR, true};
+                sg.B = {T64 + (r * o + q) * v * vK, 1, 0, vK, 0, vK, false};
             }
             g.C = dX + p * v * v * vK;
             g.sCn = 1;
             g.sCb = sx == 1 ? vK : v * vK;
             g.sCm = sx == 1 ? v * vK : vK;
-            he = launch_gemm(g, (int)v, c->stream);
+            grp.g[grp.n++] = g;
         }
+        if (he == hipSuccess)
+            he = launch_gemm_group(grp, (int)v, c->stream);
+        GemmGroup grpT{}, grpR{};
+        grpT.n = grpR.n = 0;
         for (int sgm = 0; sgm < 6 && he == hipSuccess; sgm++) {
             if (twin && (sgm == 2 || sgm == 3 || sgm == 5))
                 continue;  // the twins of terms 0, 1, 4
@@ -1022,52 +1042,58 @@ int scess_t_backward(scess_ctx *c, int prec, double dE, scess_t_grad *grad) {
                 g.N = (int)v;
                 g.alpha = alpha;
                 g.splits = PT;
-                g.scratch = scr;
+                g.scratch = scr + (size_t)grpT.n * PT * v * v;
                 GemmSeg &sg = g.seg[0];
                 sg.K1 = (int)v;
                 sg.K2 = (int)v;
                 if (sx < sy) {  // inner K over x
-                    sg.A = {GW, sz, sy, sx, 0};
-                    sg.B = {X64 + p * v * v * vK, 1, vK, v * vK, 0};
+                    sg.A = {GW, sz, sy, sx, 0, vR, true};
+                    sg.B = {X64 + p * v * v * vK, 1, vK, v * vK, 0, vK, false};
                 } else {  // inner K over y
-                    sg.A = {GW, sz, sx, sy, 0};
-                    sg.B = {X64 + p * v * v * vK, 1, v * vK, vK, 0};
+                    sg.A = {GW, sz, sx, sy, 0, vR, true};
+                    sg.B = {X64 + p * v * v * vK, 1, v * vK, vK, 0, vK, false};
                 }
                 g.C = dT + (r * o + q) * v * vK;
                 g.sCm = vK;
                 g.sCn = 1;
-                he = launch_gemm(g, 1, c->stream);
+                grpT.g[grpT.n++] = g;
             }
-            // dRn[p][q][x][m] += sum_{y,z} G(x,y,z) TM[r][y][z][m]
-            if (he == hipSuccess) {
+            // dRn[p][q][x][m] += sum_{y,z} G(x,y,z) TM[r][y][z][m] (its own group; run after the dT group,
+            // so the two reuse one scratch area)
+            {
                 GemmArgs g{};
                 g.nseg = 1;
                 g.M = (int)v;
                 g.N = (int)o;
                 g.alpha = alpha;
                 g.splits = PR;
-                g.scratch = scr;
+                g.scratch = scr + (size_t)grpR.n * PR * v * o;
                 GemmSeg &sg = g.seg[0];
                 sg.K1 = (int)v;
                 sg.K2 = (int)v;
                 if (sy < sz) {  // inner K over y
-                    sg.A = {GW, sx, sz, sy, 0};
-                    sg.B = {TM64 + r * v * v * oK, 1, oK, v * oK, 0};
+                    sg.A = {GW, sx, sz, sy, 0, vR, true};
+                    sg.B = {TM64 + r * v * v * oK, 1, oK, v * oK, 0, oK, false};
                 } else {  // inner K over z
-                    sg.A = {GW, sx, sy, sz, 0};
-                    sg.B = {TM64 + r * v * v * oK, 1, v * oK, oK, 0};
+                    sg.A = {GW, sx, sy, sz, 0, vR, true};
+                    sg.B = {TM64 + r * v * v * oK, 1, v * oK, oK, 0, oK, false};
                 }
                 g.C = dRn + (p * o + q) * v * oK;
                 g.sCm = oK;
                 g.sCn = 1;
-                he = launch_gemm(g, 1, c->stream);
+                grpR.g[grpR.n++] = g;
             }
         }
+        if (he == hipSuccess)
+            he = launch_gemm_group(grpT, 1, c->stream);
+        if (he == hipSuccess)
+            he = launch_gemm_group(grpR, 1, c->stream);
         // dTM[r][y][z][m] += sum_x Rn[p][q][x][m] G(x,y,z): the two terms with the same position for z
         // write the same dTM[r]; one GEMM with two K segments each (rows z and batch y when z is the
         // contiguous position of G, else rows y and batch z)
         // (twin: the two segments of pair 0 are equal, and pair 2 equals pair 1)
         static const int ZPAIR[3][2] = {{0, 2}, {1, 4}, {3, 5}};
+        grp.n = 0;
         for (int pr = 0; pr < 3 && he == hipSuccess; pr++) {
             if (twin && pr == 2)
                 continue;
@@ -1085,15 +1111,17 @@ int scess_t_backward(scess_ctx *c, int prec, double dE, scess_t_grad *grad) {
                 GemmSeg &sg = g.seg[h];
                 sg.K1 = 1;
                 sg.K2 = (int)v;
-                sg.B = {Rn64 + (p * o + q) * v * oK, 1, 0, oK, 0};
-                sg.A = sz == 1 ? GemmOp{GW, sz, 0, sx, sy} : GemmOp{GW, sy, 0, sx, sz};
+                sg.B = {Rn64 + (p * o + q) * v * oK, 1, 0, oK, 0, oK, false};
+                sg.A = sz == 1 ? GemmOp{GW, sz, 0, sx, sy, vR, true} : GemmOp{GW, sy, 0, sx, sz, vR, true};
             }
             g.C = dTM + r * v * v * oK;
             g.sCn = 1;
             g.sCb = sz == 1 ? v * oK : oK;
             g.sCm = sz == 1 ? oK : v * oK;
-            he = launch_gemm(g, (int)v, c->stream);
+            grp.g[grp.n++] = g;
         }
+        if (he == hipSuccess)
+            he = launch_gemm_group(grp, (int)v, c->stream);
         if (he == hipSuccess)
             he = launch_vback(GV, U, t.t1, occ[0], occ[1], occ[2], (int)o, (int)v, (int)vR, dU, dtv, dS, dt1,
                               c->stream);

@@ -13,6 +13,7 @@
 // Occupied triples are processed one after another and every kernel writes each
 // output element from one thread in a fixed order (split-K partials are summed
 // in split order), so the gradients are deterministic.
+#include <type_traits>
 #include "triples_internal.h"
 
 #include <cstdint>
@@ -411,7 +412,181 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
         }
 }
 
+// The same contraction for the layouts of the (T) back-contractions, with direct-to-LDS operand loads:
+// B has unit stride along n (rows of T2 and of the integral blocks), A (a gradient array G) unit stride
+// along m (AM = 0) or along k2 (AM = 1). A 16-deep chunk of a unit-stride-n (or m) operand is a [k][x]
+// tile of 16 rows of 64 doubles (32 sixteen-byte vectors, vector p of row k at slot p ^ 8 (k & 1)); of a
+// unit-stride-k operand a [m][k] tile of 64 rows of 16 doubles (8 vectors, vector p of row m at slot
+// p ^ ((m >> 1) & 7)). Both put the 16 lanes of every MFMA operand read on distinct banks. Loads go
+// global -> LDS (global_load_lds, 16 bytes per lane, 1 KiB per wave instruction) into a two-stage ring;
+// vectors past M or N are clamped into the padded rows (cext), B rows with k2 >= K2 to row K2 - 1 (A
+// reads zero there: kzero). Tiles, waves, MFMA blocks, splits and the store are those of gemm_kernel.
+constexpr int KN_TILE = 16 * 64;  // doubles per operand chunk (8 KiB)
+
+// source of the 16-byte vector that lane `lane` of wave load L (0..7) puts into an operand chunk
+template <int MK>
+__device__ inline const double *kn_src(const GemmOp &op, int64_t base, int x0, int xext, int k1, int k2c, int K2, int L,
+                                       int lane) {
+    if constexpr (MK) {  // [m][k] tile: wave load L covers rows 8 L .. 8 L + 7
+        const int mm = 8 * L + (lane >> 3), p = lane & 7;
+        const int lv = p ^ ((mm >> 1) & 7);
+        const int64_t k2 = min((int64_t)k2c * 16 + 2 * lv, op.cext - 2);
+        const int m = min(x0 + mm, xext - 1);
+        return op.base + base + (int64_t)m * op.sr + (int64_t)k1 * op.s1 + k2;
+    } else {  // [k][x] tile: wave load L covers rows 2 L, 2 L + 1
+        const int kk = 2 * L + (lane >> 5), p = lane & 31;
+        const int lv = p ^ ((kk & 1) << 3);
+        const int64_t x = min((int64_t)x0 + 2 * lv, op.cext - 2);
+        const int k2 = op.kzero ? k2c * 16 + kk : min(k2c * 16 + kk, K2 - 1);
+        return op.base + base + (int64_t)k1 * op.s1 + (int64_t)k2 * op.s2 + x;
+    }
+}
+
+// element (x, k) of an operand chunk in LDS (x = row of A or B within the tile, k = 0..15)
+template <int MK> __device__ inline double kn_frag(const double *t, int x, int k) {
+    if constexpr (MK)
+        return t[x * 16 + (((k >> 1) ^ ((x >> 1) & 7)) << 1) + (k & 1)];
+    else
+        return t[k * 64 + (((x >> 1) ^ ((k & 1) << 3)) << 1) + (x & 1)];
+}
+
+template <int AM0, int AM1>
+__global__ __launch_bounds__(256) void gemm_kn_kernel(GemmGroup gg) {
+    __shared__ __attribute__((aligned(1024))) double lds[2][2 * KN_TILE];  // [stage][A chunk, B chunk]
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = wave >> 1, wn = wave & 1;
+    const int nx = gridDim.x, ny = gridDim.y;
+    const int lin = xcd_remap(blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z), nx * ny * gridDim.z);
+    const int bx = lin % nx, byz = lin / nx;
+    const int beta = byz % ny, zz = byz / ny;
+    const int prob = zz % gg.n, split = zz / gg.n;
+    const GemmArgs &g = gg.g[prob];
+    const int ntn = (g.N + GT - 1) / GT;
+    const int tm = bx / ntn, tn = bx - tm * ntn;
+    const int m0 = tm * GT, n0 = tn * GT;
+
+    const int nk20 = (g.seg[0].K2 + 15) / 16, nk21 = g.nseg > 1 ? (g.seg[1].K2 + 15) / 16 : 1;
+    const int nch0 = g.seg[0].K1 * nk20, nch1 = g.nseg > 1 ? g.seg[1].K1 * nk21 : 0;
+    const int total = nch0 + nch1;
+    const int c_begin = (int)((int64_t)total * split / g.splits), c_end = (int)((int64_t)total * (split + 1) / g.splits);
+
+    d4 acc[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++)
+            acc[x][y] = d4{0.0, 0.0, 0.0, 0.0};
+    bool vm[2], vn[2];
+#pragma unroll
+    for (int x = 0; x < 2; x++) {
+        vm[x] = m0 + wm * 32 + x * 16 < g.M;
+        vn[x] = n0 + wn * 32 + x * 16 < g.N;
+    }
+
+    // wave loads 2 wave and 2 wave + 1 of each operand chunk are this wave's
+    auto issue_seg = [&](const GemmSeg &sg, auto amk, int cl, int nk2, int st) {
+        constexpr int MK = decltype(amk)::value;
+        const int k1 = cl / nk2, k2c = cl - k1 * nk2;
+        double *As = lds[st], *Bs = lds[st] + KN_TILE;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int L = 2 * wave + h;
+            const double *sa = kn_src<MK>(sg.A, (int64_t)beta * sg.A.sb, m0, g.M, k1, k2c, sg.K2, L, lane);
+            const double *sb = kn_src<0>(sg.B, (int64_t)beta * sg.B.sb, n0, g.N, k1, k2c, sg.K2, L, lane);
+            __builtin_amdgcn_global_load_lds((const void *)sa, (lds_void *)(As + L * 128), 16, 0, 0);
+            __builtin_amdgcn_global_load_lds((const void *)sb, (lds_void *)(Bs + L * 128), 16, 0, 0);
+        }
+    };
+    auto issue = [&](int ch, int st) {
+        if (ch < nch0)
+            issue_seg(g.seg[0], std::integral_constant<int, AM0>{}, ch, nk20, st);
+        else
+            issue_seg(g.seg[1], std::integral_constant<int, AM1>{}, ch - nch0, nk21, st);
+    };
+    auto compute = [&](int st, auto amk) {
+        constexpr int MK = decltype(amk)::value;
+        const double *As = lds[st], *Bs = lds[st] + KN_TILE;
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int kk = t * 4 + (lane >> 4);
+            double af[2], bf[2];
+#pragma unroll
+            for (int x = 0; x < 2; x++)
+                af[x] = kn_frag<MK>(As, wm * 32 + x * 16 + (lane & 15), kk);
+#pragma unroll
+            for (int y = 0; y < 2; y++)
+                bf[y] = kn_frag<0>(Bs, wn * 32 + y * 16 + (lane & 15), kk);
+#pragma unroll
+            for (int x = 0; x < 2; x++)
+                if (vm[x])
+#pragma unroll
+                    for (int y = 0; y < 2; y++)
+                        if (vn[y])
+                            acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[x], bf[y], acc[x][y], 0, 0, 0);
+        }
+    };
+    if (c_begin < c_end)
+        issue(c_begin, 0);
+    int st = 0;
+    for (int ch = c_begin; ch < c_end; ch++) {
+        wait_vm<0>();
+        __syncthreads();  // chunk ch is in LDS; the other stage (chunk ch - 1) is free again
+        if (ch + 1 < c_end)
+            issue(ch + 1, st ^ 1);
+        if (ch < nch0)
+            compute(st, std::integral_constant<int, AM0>{});
+        else
+            compute(st, std::integral_constant<int, AM1>{});
+        st ^= 1;
+    }
+
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++) {
+            const int n = n0 + wn * 32 + y * 16 + (lane & 15);
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int m = m0 + wm * 32 + x * 16 + (lane >> 4) + 4 * r;
+                if (m < g.M && n < g.N) {
+                    if (g.splits == 1) {
+                        double *cp = g.C + (int64_t)beta * g.sCb + (int64_t)m * g.sCm + (int64_t)n * g.sCn;
+                        *cp += g.alpha * acc[x][y][r];
+                    } else {
+                        g.scratch[((int64_t)split * g.M + m) * g.N + n] = acc[x][y][r];
+                    }
+                }
+            }
+        }
+}
+
 __global__ __launch_bounds__(256) void gemm_reduce_kernel(GemmArgs g) {
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (int64_t)g.M * g.N)
+        return;
+    const int m = (int)(idx / g.N), n = (int)(idx - (int64_t)m * g.N);
+    const int64_t mn = (int64_t)g.M * g.N;
+    const double *src = g.scratch + idx;
+    double s = 0.0;
+    int p = 0;
+    for (; p + 8 <= g.splits; p += 8) {  // eight loads in flight, added in split order
+        double x[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            x[u] = src[(int64_t)(p + u) * mn];
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            s += x[u];
+    }
+    for (; p < g.splits; p++)
+        s += src[(int64_t)p * mn];
+    g.C[(int64_t)m * g.sCm + (int64_t)n * g.sCn] += g.alpha * s;
+}
+
+// gemm_reduce_kernel for problem blockIdx.y of a group
+__global__ __launch_bounds__(256) void gemm_reduce_group_kernel(GemmGroup gg) {
+    const GemmArgs &g = gg.g[blockIdx.y];
     const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (idx >= (int64_t)g.M * g.N)
         return;
@@ -446,6 +621,7 @@ __device__ inline int kswz(int r) { return (r ^ (r >> 3)) & 7; }
 
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef short s4v __attribute__((ext_vector_type(4)));
+typedef __bf16 b8v __attribute__((ext_vector_type(8)));
 
 __device__ inline short bf16_bits(float f) {  // round to nearest even
     uint32_t u = __float_as_uint(f);
@@ -571,6 +747,32 @@ __global__ __launch_bounds__(256) void gemm_kc_kernel(GemmArgs g) {
                                     acc[x][y] =
                                         __builtin_amdgcn_mfma_f64_16x16x4f64(af[x][t], bf[y][t], acc[x][y], 0, 0, 0);
             }
+        } else if constexpr (PREC == PB3S) {
+            // a 16-slot chunk holds 32 k of pre-split rows: vector 2 lq = the hi bf16 of k = 8 lq .. 8 lq + 7,
+            // vector 2 lq + 1 their lo bf16 (the 16x16x32 operand layout); hi*hi + hi*lo + lo*hi in FP32 per
+            // chunk, added to the FP64 accumulators
+            b8v ah[BLK], al[BLK], bh[BLK], bl[BLK];
+#pragma unroll
+            for (int b = 0; b < BLK; b++) {
+                const int rowa = wm * BLK * 16 + b * 16 + lr, rowb = wn * BLK * 16 + b * 16 + lr;
+                ah[b] = *(const b8v *)(As + rowa * RB + (((2 * lq) ^ kswz(rowa)) << 4));
+                al[b] = *(const b8v *)(As + rowa * RB + (((2 * lq + 1) ^ kswz(rowa)) << 4));
+                bh[b] = *(const b8v *)(Bs + rowb * RB + (((2 * lq) ^ kswz(rowb)) << 4));
+                bl[b] = *(const b8v *)(Bs + rowb * RB + (((2 * lq + 1) ^ kswz(rowb)) << 4));
+            }
+#pragma unroll
+            for (int x = 0; x < BLK; x++)
+#pragma unroll
+                for (int y = 0; y < BLK; y++) {
+                    if (!vm[x] || !vn[y])
+                        continue;
+                    f4v c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[x], bh[y], f4v{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bl[y], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bh[y], c, 0, 0, 0);
+#pragma unroll
+                    for (int r = 0; r < 4; r++)
+                        acc[x][y][r] += (double)c[r];
+                }
         } else {
             // the lane's 4 k values of each block row, rounded (FP32) or split (hi + lo BF16)
             f4v a32[BLK], b32[BLK];
@@ -765,6 +967,39 @@ static int pair_mode(const GemmOp &op) {
     return 0;
 }
 
+// Whether gemm_kn_kernel applies (FP64 products, B of unit stride in n, A of unit stride in m or in
+// k2, every stride and base even so that operand vectors are 16-byte aligned, padded extents known, A
+// zero past K2): 0 if not, else 1 + (segment 0's A unit-stride in k2) + 2 (segment 1's).
+// SCESS_GEMM_KN=0 keeps the register-staged kernel.
+static int kn_modes(const GemmArgs &g) {
+    if (g.prec != P64)
+        return 0;
+    if (const char *e = std::getenv("SCESS_GEMM_KN"))
+        if (std::atoi(e) == 0)
+            return 0;
+    auto even = [](const GemmOp &op) {
+        return !(op.sr & 1) || op.sr == 1 ? !(op.s1 & 1) && !(op.sb & 1) && !(((uintptr_t)op.base) & 15) : false;
+    };
+    int am = 1;
+    for (int q = 0; q < g.nseg; q++) {
+        const GemmOp &A = g.seg[q].A, &B = g.seg[q].B;
+        if (B.sr != 1 || (B.s2 & 1) || B.cext < 2 || (B.cext & 1) || !even(B) || g.seg[q].K2 < 1 || g.seg[q].K1 < 1)
+            return 0;
+        if (!A.kzero || A.cext < 2 || (A.cext & 1) || !even(A))
+            return 0;
+        if (A.sr == 1 && !(A.s2 & 1))
+            continue;  // [k][m] tile
+        if (A.s2 == 1 && !(A.sr & 1) && A.cext >= (g.seg[q].K2 + 15) / 16 * 16) {
+            am += q == 0 ? 1 : 2;  // [m][k] tile
+            continue;
+        }
+        return 0;
+    }
+    if (g.nseg == 1 && am == 2)
+        am = 4;  // segment 1 is not used: give it segment 0's mode
+    return am;
+}
+
 hipError_t launch_gemm(const GemmArgs &g, int batch, hipStream_t s) {
     {  // the kernels index K chunks (16 deep) with 32-bit integers
         int64_t chunks = 0;
@@ -793,11 +1028,17 @@ hipError_t launch_gemm(const GemmArgs &g, int batch, hipStream_t s) {
         hipLaunchKernelGGL((gemm_kc_kernel<B, V, P, D>), grid, dim3(256), 0, s, g);               \
     } else
         SCESS_KC(2, true, P64, true) SCESS_KC(2, true, P32, true) SCESS_KC(2, true, PB3, true)
+        SCESS_KC(2, true, PB3S, true) SCESS_KC(2, true, PB3S, false)
         SCESS_KC(2, true, P64, false) SCESS_KC(2, false, P64, false) SCESS_KC(2, true, P32, false)
         SCESS_KC(2, false, P32, false) SCESS_KC(2, true, PB3, false) SCESS_KC(2, false, PB3, false)
         SCESS_KC(4, true, P64, false) SCESS_KC(4, false, P64, false)
         return hipErrorInvalidValue;
 #undef SCESS_KC
+    } else if (kn_modes(g)) {
+        GemmGroup gg;
+        gg.g[0] = g;
+        gg.n = 1;
+        return launch_gemm_group(gg, batch, s);
     } else {
         const int ntiles = ((g.M + GT - 1) / GT) * ((g.N + GT - 1) / GT);
         const dim3 grid((unsigned)ntiles, (unsigned)batch, (unsigned)g.splits);
@@ -820,6 +1061,67 @@ hipError_t launch_gemm(const GemmArgs &g, int batch, hipStream_t s) {
     const int64_t n = (int64_t)g.M * g.N;
     hipLaunchKernelGGL(gemm_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g);
     return hipGetLastError();
+}
+
+static hipError_t launch_kn_group(const GemmGroup &gg, int am, int batch, hipStream_t s) {
+    const GemmArgs &g = gg.g[0];
+    const int ntiles = ((g.M + GT - 1) / GT) * ((g.N + GT - 1) / GT);
+    const dim3 grid((unsigned)ntiles, (unsigned)batch, (unsigned)(g.splits * gg.n));
+    // am: 1 + (A of segment 0 unit-stride in k) + 2 (A of segment 1 unit-stride in k)
+    switch (am) {
+    case 1: hipLaunchKernelGGL((gemm_kn_kernel<0, 0>), grid, dim3(256), 0, s, gg); break;
+    case 2: hipLaunchKernelGGL((gemm_kn_kernel<1, 0>), grid, dim3(256), 0, s, gg); break;
+    case 3: hipLaunchKernelGGL((gemm_kn_kernel<0, 1>), grid, dim3(256), 0, s, gg); break;
+    default: hipLaunchKernelGGL((gemm_kn_kernel<1, 1>), grid, dim3(256), 0, s, gg); break;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || g.splits == 1)
+        return e;
+    const int64_t n = (int64_t)g.M * g.N;
+    hipLaunchKernelGGL(gemm_reduce_group_kernel, dim3((unsigned)((n + 255) / 256), (unsigned)gg.n), dim3(256), 0, s,
+                       gg);
+    return hipGetLastError();
+}
+
+hipError_t launch_gemm_group(const GemmGroup &gg, int batch, hipStream_t s) {
+    if (gg.n < 1 || gg.n > kMaxGemmGroup)
+        return hipErrorInvalidValue;
+    const GemmArgs &g0 = gg.g[0];
+    for (int p = 1; p < gg.n; p++)  // one shape per group
+        if (gg.g[p].M != g0.M || gg.g[p].N != g0.N || gg.g[p].splits != g0.splits)
+            return hipErrorInvalidValue;
+    for (int p = 0; p < gg.n; p++) {
+        int64_t chunks = 0;  // the kernels index K chunks (16 deep) with 32-bit integers
+        for (int q = 0; q < gg.g[p].nseg; q++)
+            chunks += (int64_t)gg.g[p].seg[q].K1 * ((gg.g[p].seg[q].K2 + 15) / 16);
+        if (chunks >= (int64_t)1 << 31 || gg.g[p].nseg < 1 || gg.g[p].nseg > 2)
+            return hipErrorInvalidValue;
+    }
+    // one launch per operand layout class (problems keep their relative order; outputs are disjoint)
+    bool done[kMaxGemmGroup] = {};
+    for (int p = 0; p < gg.n; p++) {
+        if (done[p])
+            continue;
+        const int am = kn_modes(gg.g[p]);
+        if (!am) {
+            hipError_t e = launch_gemm(gg.g[p], batch, s);  // not a direct-to-LDS layout
+            if (e != hipSuccess)
+                return e;
+            done[p] = true;
+            continue;
+        }
+        GemmGroup sub;
+        sub.n = 0;
+        for (int q = p; q < gg.n; q++)
+            if (!done[q] && kn_modes(gg.g[q]) == am) {
+                sub.g[sub.n++] = gg.g[q];
+                done[q] = true;
+            }
+        hipError_t e = launch_kn_group(sub, am, batch, s);
+        if (e != hipSuccess)
+            return e;
+    }
+    return hipSuccess;
 }
 
 // ------------------------------------------------------------- V backward

@@ -10,6 +10,10 @@ namespace scess {
 
 enum Prec : int { P64 = 0, P32 = 1, PB3 = 2, PI8 = 3 };
 constexpr int kNumPrec = 4;
+// GemmArgs::prec only: BF16x3 products of operands stored pre-split (split_rows_kernel layout: per 8 k,
+// 8 hi then 8 lo bf16; rows padded to a multiple of 32 k with zeros). Such an operand is passed as
+// FP64 rows of K/2 slots (the same bytes), so K2 and the strides count 8-byte slots.
+constexpr int PB3S = 4;
 
 // K-dimension chunk (elements) of each precision's W-build kernel; the K
 // extents of the packed operands are padded to a multiple of it with zeros.
@@ -142,6 +146,11 @@ hipError_t launch_fill_zero(double *p, int64_t n, hipStream_t s);
 struct GemmOp {
     const double *base;
     int64_t sr, s1, s2, sb;
+    // Elements readable along the unit-stride dimension from index 0 (the padded row length of the
+    // array; 0 = unknown), and whether entries with k2 in [K2, K2 rounded up to 16) read as zero. With
+    // these known, the back-contraction GEMMs use direct-to-LDS loads (gemm_kn_kernel).
+    int64_t cext;
+    bool kzero;
 };
 struct GemmSeg {
     GemmOp A, B;  // A rows m, B rows n
@@ -163,6 +172,16 @@ struct GemmArgs {
     int prec;
 };
 hipError_t launch_gemm(const GemmArgs &g, int batch, hipStream_t s);
+// Up to kMaxGemmGroup GEMMs of one shape (M, N, batch, splits) writing disjoint outputs, launched
+// together (direct-to-LDS kernel: grid z = split x problem; grouped by operand layout) or, where that
+// kernel does not apply, one after another in the given order. Each problem's splits > 1 needs its own
+// scratch. The sums are those of launch_gemm on each problem.
+constexpr int kMaxGemmGroup = 6;
+struct GemmGroup {
+    GemmArgs g[kMaxGemmGroup];
+    int n;
+};
+hipError_t launch_gemm_group(const GemmGroup &gg, int batch, hipStream_t s);
 // G_W, G_V ([vR]^3 row-major, FP64) of the single occupied triple a.ijk; W in a.W1/W2, V pieces in a.U
 hipError_t launch_adjoint_g(int prec, const EpiArgs &a, double wgt, double *GW, double *GV, hipStream_t s);
 // V backward of one occupied triple into dS [o][o][v][v] and dt1 [o][vR]; dU (3 v^2) and dt (3 v) are scratch
