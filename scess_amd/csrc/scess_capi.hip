@@ -746,7 +746,9 @@ int scess_t_forward_ex(scess_ctx *c, int prec, double *e_t, double *F_ijk, doubl
         }
         stats->flops_algorithmic = 2.0 * (double)o * o * o * (double)v * v * v * (double)(o + v) *
                                    (ordered_all > 0.0 ? ordered / ordered_all : 1.0);
-        const double rows = (double)round_up(v, 16);
+        // rows and columns as the kernels issue them: MFMA blocks of 16 (blocks wholly past v skipped);
+        // the int8-digit kernel computes every 32-row block of its 64 x 64 tiles
+        const double rows = (double)round_up(v, prec == PI8 ? 64 : 16);
         stats->flops_executed = ((double)nshare - 0.5 * (double)ndeg) * 2.0 * (double)v * rows * rows *
                                 (6.0 * (double)op.vK + 6.0 * (double)op.oK);
     }
