@@ -883,11 +883,25 @@ int scess_t_backward(scess_ctx *c, int prec, double dE, scess_t_grad *grad) {
         }
         return e;
     };
+    // the W of the next batch of triples is built on a second stream into the other of two W sets,
+    // while the G kernels and back-contractions of the current batch run on c->stream
+    hipStream_t ws = nullptr;
+    hipEvent_t evW[2] = {nullptr, nullptr}, evDone[2] = {nullptr, nullptr};
     auto release = [&]() {
         (void)hipStreamSynchronize(c->stream);
+        if (ws)
+            (void)hipStreamSynchronize(ws);
         for (void *p : bufs)
             (void)hipFree(p);
         bufs.clear();
+        for (int q = 0; q < 2; q++) {
+            if (evW[q])
+                (void)hipEventDestroy(evW[q]);
+            if (evDone[q])
+                (void)hipEventDestroy(evDone[q]);
+        }
+        if (ws)
+            (void)hipStreamDestroy(ws);
     };
     // K splits of dT (v x v outputs, K = v^2) and dRn (v x o outputs, K = v^2): about 1024 work-groups per
     // GEMM so that the long K loops overlap their memory latency across the chip (the terms of a triple
@@ -903,8 +917,8 @@ int scess_t_backward(scess_ctx *c, int prec, double dE, scess_t_grad *grad) {
         }
     }
     double *dX, *dT, *dRn, *dTM, *dS, *dt1, *GW, *GV, *U, *dU, *dtv, *scr, *out;
-    void *W1, *W2;
-    // W of up to WB triples is built by one launch (2 buffers of WB x vR^3 within 4 GB)
+    void *W1[2], *W2[2];
+    // W of up to WB triples is built by one launch (2 buffers of WB x vR^3 within 4 GB per set)
     const int64_t WB = std::max<int64_t>(1, std::min<int64_t>(8, (int64_t)(4.0e9 / (2.0 * (double)wstride * accb))));
     hipError_t he = hipSuccess;
     const size_t nX = (size_t)(o * v * v * vK), nT = (size_t)(o * o * v * vK), nR = (size_t)(o * o * v * oK),
@@ -922,20 +936,35 @@ int scess_t_backward(scess_ctx *c, int prec, double dE, scess_t_grad *grad) {
         (he = alloc(8 * 3 * (size_t)(v * v), (void **)&dU)) != hipSuccess ||
         (he = alloc(8 * 3 * (size_t)v, (void **)&dtv)) != hipSuccess ||
         (he = alloc(8 * nscr, (void **)&scr)) != hipSuccess || (he = alloc(8 * nout, (void **)&out)) != hipSuccess ||
-        (he = alloc(accb * (size_t)(WB * wstride), &W1)) != hipSuccess ||
-        (he = alloc(accb * (size_t)(WB * wstride), &W2)) != hipSuccess) {
+        (he = alloc(accb * (size_t)(WB * wstride), &W1[0])) != hipSuccess ||
+        (he = alloc(accb * (size_t)(WB * wstride), &W2[0])) != hipSuccess ||
+        (he = alloc(accb * (size_t)(WB * wstride), &W1[1])) != hipSuccess ||
+        (he = alloc(accb * (size_t)(WB * wstride), &W2[1])) != hipSuccess) {
         release();
         return hip_fail(c, he, "scess_t_backward: device allocation");
+    }
+    if ((he = hipStreamCreateWithFlags(&ws, hipStreamNonBlocking)) != hipSuccess) {
+        ws = nullptr;
+        release();
+        return hip_fail(c, he, "scess_t_backward: stream");
+    }
+    for (int q = 0; q < 2 && he == hipSuccess; q++) {
+        he = hipEventCreateWithFlags(&evW[q], hipEventDisableTiming);
+        if (he == hipSuccess)
+            he = hipEventCreateWithFlags(&evDone[q], hipEventDisableTiming);
+    }
+    // the W set zeroed by alloc on c->stream must be clear before the second stream writes it
+    if (he == hipSuccess)
+        he = hipStreamSynchronize(c->stream);
+    if (he != hipSuccess) {
+        release();
+        return hip_fail(c, he, "scess_t_backward: events");
     }
 
     const int blk = wbuild_tile(prec, (int)v);
     // triples with i == j build only W1; their W2 is read from W1 (as the forward epilogue)
     WBuildArgs wa = wbuild_args(t, prec, blk, wstride);
-    wa.W1 = W1;
-    wa.W2 = W2;
     EpiArgs ea{};
-    ea.W1 = W1;
-    ea.W2 = W2;
     ea.wstride = wstride;
     ea.o = (int)o;
     ea.v = (int)v;
@@ -972,14 +1001,34 @@ int scess_t_backward(scess_ctx *c, int prec, double dE, scess_t_grad *grad) {
         const int occ[3] = {t.work_ijk[3 * w], t.work_ijk[3 * w + 1], t.work_ijk[3 * w + 2]};
         const double deg = (occ[0] == occ[1] || occ[1] == occ[2]) ? 2.0 : 1.0;
         const int *dijk = t.d_work_ijk + 3 * w;
-        const int64_t slot = (w - w_begin) % WB;
-        if (slot == 0) {  // W of the next WB triples
-            wa.ijk = dijk;
-            he = launch_wbuild(prec, blk, wbuild_dma(), wa, (int)std::min<int64_t>(WB, w_end - w), c->stream);
+        const int64_t slot = (w - w_begin) % WB, batch = (w - w_begin) / WB, set = batch & 1;
+        if (slot == 0) {
+            // W of batch b + 1 on the second stream (W of batch 0 first), once batch b - 1 has released
+            // that set; batch b waits for its own W
+            auto build = [&](int64_t b) {
+                const int64_t w0 = w_begin + b * WB;
+                hipError_t e = hipSuccess;
+                if (b >= 2)
+                    e = hipStreamWaitEvent(ws, evDone[b & 1], 0);
+                wa.ijk = t.d_work_ijk + 3 * w0;
+                wa.W1 = W1[b & 1];
+                wa.W2 = W2[b & 1];
+                if (e == hipSuccess)
+                    e = launch_wbuild(prec, blk, wbuild_dma(), wa, (int)std::min<int64_t>(WB, w_end - w0), ws);
+                if (e == hipSuccess)
+                    e = hipEventRecord(evW[b & 1], ws);
+                return e;
+            };
+            if (batch == 0)
+                he = build(0);
+            if (he == hipSuccess && w + WB < w_end)
+                he = build(batch + 1);
+            if (he == hipSuccess)
+                he = hipStreamWaitEvent(c->stream, evW[set], 0);
         }
         ea.ijk = dijk;
-        ea.W1 = (const char *)W1 + (size_t)slot * wstride * accb;
-        ea.W2 = occ[0] == occ[1] ? ea.W1 : (const char *)W2 + (size_t)slot * wstride * accb;
+        ea.W1 = (const char *)W1[set] + (size_t)slot * wstride * accb;
+        ea.W2 = occ[0] == occ[1] ? ea.W1 : (const char *)W2[set] + (size_t)slot * wstride * accb;
         if (he == hipSuccess)
             he = launch_vpieces(t.S, dijk, (int)o, (int)v, (int)vR, U, 1, c->stream);
         // G_W entries with an index in [v, vR) are zero: the buffer starts zeroed and the G kernel writes
@@ -1127,6 +1176,9 @@ int scess_t_backward(scess_ctx *c, int prec, double dE, scess_t_grad *grad) {
         if (he == hipSuccess)
             he = launch_vback(GV, U, t.t1, occ[0], occ[1], occ[2], (int)o, (int)v, (int)vR, dU, dtv, dS, dt1,
                               c->stream);
+        // the last triple of a batch releases its W set
+        if (he == hipSuccess && (slot == WB - 1 || w == w_end - 1))
+            he = hipEventRecord(evDone[set], c->stream);
         if (he != hipSuccess)
             rc = hip_fail(c, he, "scess_t_backward: kernel launch");
     }
