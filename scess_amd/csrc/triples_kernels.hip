@@ -985,8 +985,8 @@ static hipError_t launch_wbuild_p(int blk, bool dma, const WBuildArgs &a, int nb
     if (dma && (blk == 2 || blk == 4 || (blk == 5 && PREC != P32))) {
         // two stages: more resident work-groups per CU beat deeper prefetch (sweeps at v = 300)
         int st = 2;
-        if (const char *e = std::getenv("SCESS_WBUILD_STAGES"))  // tuning override: 2, 3 or 4
-            st = std::atoi(e) >= 2 && std::atoi(e) <= 4 ? std::atoi(e) : st;
+        if (const char *e = std::getenv("SCESS_WBUILD_STAGES"))  // tuning override: 2, 3 or 4 (3 at most for blk 5)
+            st = std::atoi(e) >= 2 && std::atoi(e) <= (blk == 5 ? 3 : 4) ? std::atoi(e) : st;
 #define SCESS_DMA_CASE(B, S)                                                                     \
     if (blk == B && st == S) {                                                                   \
         hipLaunchKernelGGL((wbuild_dma_kernel<PREC, B, S>), grid, dim3(256), 0, s, a);           \
@@ -1017,6 +1017,13 @@ hipError_t launch_wbuild(int prec, int blk, bool dma, const WBuildArgs &a, int n
             nst = std::atoi(e) == 3 ? 3 : 2;
         if (const char *e = std::getenv("SCESS_I8_FX"))  // tuning override: 1, 2 or 4
             f = std::atoi(e);
+        // overrides without an instantiation below fall back to the defaults
+        const bool built = (f == 1 || f == 2 || (f == 4 && a.slices <= 6)) &&
+                           (nst == 2 || (nst == 3 && f == 4 && a.slices == 5));
+        if (!built) {
+            f = i8_fixed_per_group(a.slices);
+            nst = 2;
+        }
         const int nfg = (a.v + f - 1) / f;
         const dim3 grid((unsigned)(2 * nfg * a.nt * a.nt), (unsigned)nbatch);
 #define SCESS_I8F_CASE(S_, F_, N_)                                                               \
