@@ -715,6 +715,184 @@ __global__ __launch_bounds__(256) void wbuild_dma_kernel(WBuildArgs a) {
         }
 }
 
+// Whether the fixed index of segment s (get_seg numbering) sits on operand A (else on B).
+__device__ inline bool seg_fx_on_a(bool fam1, int s) { return fam1 && (s == 0 || s == 2 || s == 6 || s == 7); }
+
+// ------------------------------- W build, FP64, two fixed indices per work-group
+// wbuild_dma_kernel<P64, 2, STAGES> with two outputs per work-group: the 64 x 64 tile (tr, tc) of W
+// for the fixed indices fx0 = 2 g and fx0 + 1. In every segment exactly one operand depends on the
+// fixed index (seg_fx_on_a), so a stage holds the other operand's tile once (tile 0) and the
+// fixed-index operand's tile of each fixed index (tiles 1, 2): 24 KiB per stage where two work-groups
+// of wbuild_dma_kernel load 32 KiB, and 32 MFMAs per wave between barriers instead of 16. Wave w
+// computes fixed index w >> 1, columns 32 (w & 1) .. + 31 of the tile, all 64 rows (eight 16 x 16
+// blocks). The fragments and the order of the MFMAs into each accumulator are those of
+// wbuild_dma_kernel, so W is bitwise the same.
+template <int STAGES>
+__global__ __launch_bounds__(256) void wbuild_f2_kernel(WBuildArgs a) {
+    constexpr int RB = 128, VPR = 8;  // 16 doubles per row chunk, as eight 16-byte vectors
+    constexpr int OPB = 64 * RB;      // bytes of one operand tile chunk (8 KiB)
+    constexpr int NLW = 3 * OPB / 1024 / 4;  // 1 KiB wave loads per wave per stage (6)
+    static_assert(STAGES == 2 || STAGES == 3, "stages");
+    __shared__ __attribute__((aligned(1024))) char lds[STAGES][3 * OPB];
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wf = wave >> 1, wp = wave & 1;  // fixed index (0, 1) and column half of this wave
+    const int lr = lane & 15, lq = lane >> 4;
+
+    const int nt2 = a.nt * a.nt;
+    const int nfg = (a.v + 1) >> 1;  // fixed-index pairs per family
+    const int gx = gridDim.x;
+    const int lin = xcd_remap(blockIdx.y * gx + blockIdx.x, gx * gridDim.y);
+    const int bat = lin / gx;
+    int bid = lin - bat * gx;
+    const bool fam1 = bid < nfg * nt2;
+    if (!fam1)
+        bid -= nfg * nt2;
+    const int fg = bid / nt2;
+    const int tt = bid - fg * nt2;
+    const int tr = tt / a.nt, tc = tt - tr * a.nt;
+    const int *q = a.ijk + 3 * bat;
+    const int64_t i = q[0], j = q[1], k = q[2];
+    const int v = a.v;
+    const int r0 = tr * 64, c0 = tc * 64;
+    const int64_t fx0 = 2 * (int64_t)fg;
+    const int64_t fx1 = min(fx0 + 1, (int64_t)v - 1);  // with v odd the last pair's second index loads v - 1
+    const bool live = fx0 + wf < v;                    // and its waves compute and store nothing
+
+    const bool deg = a.halve && i == j;
+    if (deg && !fam1)
+        return;  // the whole work-group: W2 of this entry is not used
+    const int nseg = fam1 ? (deg ? 6 : 8) : 4;
+    const int total = fam1 ? (deg ? 3 : 4) * (a.nchv + a.ncho) : 2 * (a.nchv + a.ncho);
+
+    bool vr[4], vc[2];
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+        vr[x] = r0 + x * 16 < v;
+#pragma unroll
+    for (int y = 0; y < 2; y++)
+        vc[y] = c0 + wp * 32 + y * 16 < v;
+
+    d4 acc[4][2];
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++)
+            acc[x][y] = d4{};
+
+    // wave load L = wave * NLW + u fills LDS KiB L of a stage: tile L / 8, rows 8 (L % 8) .. + 7; the
+    // lane's row in its tile and the byte offset of the logical vector it fetches (fixed per lane)
+    int lrow[NLW], lvoff[NLW];
+#pragma unroll
+    for (int u = 0; u < NLW; u++) {
+        const int L = wave * NLW + u;
+        const int r = (L & 7) * 8 + (lane >> 3), p = lane & 7;
+        lrow[u] = r;
+        lvoff[u] = (p ^ lds_swz<VPR>(r)) * 16;
+    }
+
+    // this lane's load addresses at k-chunk 0 of segment sgi (rows past v load row v - 1, valid memory
+    // whose outputs are never stored); a chunk adds kc * RB
+    const char *pl[NLW];
+    int nch = 0;
+    bool fa = false;
+    auto set_seg = [&](int sgi) {
+        const Seg s0 = get_seg(a, fam1, sgi, i, j, k, fx0);
+        const Seg s1 = get_seg(a, fam1, sgi, i, j, k, fx1);
+        fa = seg_fx_on_a(fam1, sgi);
+        nch = s0.nch;
+#pragma unroll
+        for (int u = 0; u < NLW; u++) {
+            const int tl = (wave * NLW + u) >> 3;  // wave-uniform
+            const Seg &sx = tl == 2 ? s1 : s0;
+            // tile 0: the operand without the fixed index; tiles 1, 2: the one with it
+            const bool use_a = (tl == 0) != fa;
+            const char *base = use_a ? sx.A : sx.B;
+            const int64_t ld = use_a ? sx.lda : sx.ldb;
+            const int row = min((use_a ? r0 : c0) + lrow[u], v - 1);
+            pl[u] = base + row * ld + lvoff[u];
+        }
+    };
+    set_seg(0);
+    int s = 0, kc = 0, famask = 0;  // famask bit st: the chunk in stage st has the fixed index on A
+    auto issue = [&](int st) {
+        char *base = lds[st];
+        const int64_t off = (int64_t)kc * RB;
+#pragma unroll
+        for (int u = 0; u < NLW; u++)
+            __builtin_amdgcn_global_load_lds((const void *)(pl[u] + off),
+                                             (lds_void *)(base + (wave * NLW + u) * 1024), 16, 0, 0);
+        famask = fa ? famask | (1 << st) : famask & ~(1 << st);
+        if (++kc == nch) {
+            kc = 0;
+            if (++s < nseg)
+                set_seg(deg ? halved_seg(s) : s);
+        }
+    };
+    auto frag = [&](const char *op, int rr, int lv) { return op + rr * RB + ((lv ^ lds_swz<VPR>(rr)) * 16); };
+    auto compute = [&](int st) {
+        const bool sa = (famask >> st) & 1;
+        const char *T0 = lds[st], *Tf = lds[st] + (1 + wf) * OPB;
+        const char *As = sa ? Tf : T0, *Bs = sa ? T0 : Tf;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {  // two k-pairs per 16-byte vector
+            d2 af[4], bf[2];
+#pragma unroll
+            for (int x = 0; x < 4; x++)
+                af[x] = *(const d2 *)frag(As, x * 16 + lr, 2 * lq + h);
+#pragma unroll
+            for (int y = 0; y < 2; y++)
+                bf[y] = *(const d2 *)frag(Bs, wp * 32 + y * 16 + lr, 2 * lq + h);
+#pragma unroll
+            for (int t = 0; t < 2; t++)
+#pragma unroll
+                for (int x = 0; x < 4; x++)
+                    if (vr[x])
+#pragma unroll
+                        for (int y = 0; y < 2; y++)
+                            if (vc[y])
+                                acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[x][t], bf[y][t], acc[x][y], 0, 0, 0);
+        }
+    };
+
+#pragma unroll
+    for (int st = 0; st < STAGES - 1; st++)
+        if (st < total)
+            issue(st);
+    for (int c = 0; c < total; c++) {
+        // this wave's loads of chunk c have landed once at most `ahead` later chunks are outstanding
+        if (STAGES > 2 && c + 1 < total)
+            wait_vm<(STAGES > 2 ? NLW : 0)>();
+        else
+            wait_vm<0>();
+        __syncthreads();  // every wave's chunk c is in LDS; stage (c - 1) % STAGES is free again
+        if (c + STAGES - 1 < total)
+            issue((c + STAGES - 1) % STAGES);
+        if (live)
+            compute(c % STAGES);
+    }
+    if (!live)
+        return;
+
+    double *W = (double *)(fam1 ? a.W1 : a.W2) + (int64_t)bat * a.wstride;
+    const int nbR = a.vR >> 3;
+    const int64_t fx = fx0 + wf;
+    const int64_t fbase = ((int64_t)(fx >> 3) * nbR) * nbR * 512 + (fx & 7) * 64;
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++) {
+            const int col = c0 + wp * 32 + y * 16 + lr;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = r0 + x * 16 + lq + 4 * r;
+                if (row < v && col < v)
+                    W[fbase + ((int64_t)(row >> 3) * nbR + (col >> 3)) * 512 + (row & 7) * 8 + (col & 7)] = acc[x][y][r];
+            }
+        }
+}
+
 // ------------------------------------------------------ W build, int8 digits
 // Exact-product mode (PI8). Every operand element is S signed 8-bit digits (pack_i8_kernel); the
 // product of two elements is sum_{t,u} d_t d'_u 256^-(t+u) times the two tensor scales. The leading
@@ -747,8 +925,6 @@ template <int S> __device__ inline int i8_swz(int r) {
     else
         return ((r >> 2) ^ (r >> 3)) & 1;
 }
-
-__device__ inline bool seg_fx_on_a(bool fam1, int s) { return fam1 && (s == 0 || s == 2 || s == 6 || s == 7); }
 
 template <int F> constexpr int i8f_waves() { return F == 4 ? 8 : 4; }
 
@@ -982,6 +1158,20 @@ int wbuild_tile(int prec, int v) {
 template <int PREC>
 static hipError_t launch_wbuild_p(int blk, bool dma, const WBuildArgs &a, int nbatch, hipStream_t s) {
     const dim3 grid((unsigned)(2 * a.ntile1), (unsigned)nbatch);
+    if constexpr (PREC == P64) {
+        // FP64 64 x 64 tiles: two fixed indices per work-group (SCESS_WBUILD_F2=0: one, tuning override)
+        const char *e = std::getenv("SCESS_WBUILD_F2");
+        if (dma && blk == 2 && !(e && std::atoi(e) == 0)) {
+            const int nfg = (a.v + 1) / 2;
+            const dim3 g2((unsigned)(2 * nfg * a.nt * a.nt), (unsigned)nbatch);
+            const char *es = std::getenv("SCESS_WBUILD_STAGES");  // tuning override: 2 or 3
+            if (es && std::atoi(es) == 3)
+                hipLaunchKernelGGL((wbuild_f2_kernel<3>), g2, dim3(256), 0, s, a);
+            else
+                hipLaunchKernelGGL((wbuild_f2_kernel<2>), g2, dim3(256), 0, s, a);
+            return hipGetLastError();
+        }
+    }
     if (dma && (blk == 2 || blk == 4 || (blk == 5 && PREC != P32))) {
         // two stages: more resident work-groups per CU beat deeper prefetch (sweeps at v = 300)
         int st = 2;
