@@ -450,9 +450,10 @@ template <int MK> __device__ inline double kn_frag(const double *t, int x, int k
         return t[k * 64 + (((x >> 1) ^ ((k & 1) << 3)) << 1) + (x & 1)];
 }
 
-template <int AM0, int AM1>
+template <int AM0, int AM1, int NST>
 __global__ __launch_bounds__(256) void gemm_kn_kernel(GemmGroup gg) {
-    __shared__ __attribute__((aligned(1024))) double lds[2][2 * KN_TILE];  // [stage][A chunk, B chunk]
+    static_assert(NST == 2 || NST == 3, "stages");
+    __shared__ __attribute__((aligned(1024))) double lds[NST][2 * KN_TILE];  // [stage][A chunk, B chunk]
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = wave >> 1, wn = wave & 1;
@@ -526,19 +527,25 @@ __global__ __launch_bounds__(256) void gemm_kn_kernel(GemmGroup gg) {
                             acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[x], bf[y], acc[x][y], 0, 0, 0);
         }
     };
-    if (c_begin < c_end)
-        issue(c_begin, 0);
+#pragma unroll
+    for (int p = 0; p < NST - 1; p++)
+        if (c_begin + p < c_end)
+            issue(c_begin + p, p);
     int st = 0;
     for (int ch = c_begin; ch < c_end; ch++) {
-        wait_vm<0>();
-        __syncthreads();  // chunk ch is in LDS; the other stage (chunk ch - 1) is free again
-        if (ch + 1 < c_end)
-            issue(ch + 1, st ^ 1);
+        // this wave's four loads per chunk: chunk ch has landed once only later chunks' are outstanding
+        if (NST > 2 && ch + 1 < c_end)
+            wait_vm<(NST > 2 ? 4 : 0)>();
+        else
+            wait_vm<0>();
+        __syncthreads();  // chunk ch is in LDS; the stage of chunk ch - 1 is free again
+        if (ch + NST - 1 < c_end)
+            issue(ch + NST - 1, st == 0 ? NST - 1 : st - 1);
         if (ch < nch0)
             compute(st, std::integral_constant<int, AM0>{});
         else
             compute(st, std::integral_constant<int, AM1>{});
-        st ^= 1;
+        st = st == NST - 1 ? 0 : st + 1;
     }
 
 #pragma unroll
@@ -1068,11 +1075,21 @@ static hipError_t launch_kn_group(const GemmGroup &gg, int am, int batch, hipStr
     const int ntiles = ((g.M + GT - 1) / GT) * ((g.N + GT - 1) / GT);
     const dim3 grid((unsigned)ntiles, (unsigned)batch, (unsigned)(g.splits * gg.n));
     // am: 1 + (A of segment 0 unit-stride in k) + 2 (A of segment 1 unit-stride in k)
-    switch (am) {
-    case 1: hipLaunchKernelGGL((gemm_kn_kernel<0, 0>), grid, dim3(256), 0, s, gg); break;
-    case 2: hipLaunchKernelGGL((gemm_kn_kernel<1, 0>), grid, dim3(256), 0, s, gg); break;
-    case 3: hipLaunchKernelGGL((gemm_kn_kernel<0, 1>), grid, dim3(256), 0, s, gg); break;
-    default: hipLaunchKernelGGL((gemm_kn_kernel<1, 1>), grid, dim3(256), 0, s, gg); break;
+    const char *e3 = std::getenv("SCESS_KN_STAGES");  // tuning override: 2 or 3 LDS stages
+    if (e3 && std::atoi(e3) == 3) {
+        switch (am) {
+        case 1: hipLaunchKernelGGL((gemm_kn_kernel<0, 0, 3>), grid, dim3(256), 0, s, gg); break;
+        case 2: hipLaunchKernelGGL((gemm_kn_kernel<1, 0, 3>), grid, dim3(256), 0, s, gg); break;
+        case 3: hipLaunchKernelGGL((gemm_kn_kernel<0, 1, 3>), grid, dim3(256), 0, s, gg); break;
+        default: hipLaunchKernelGGL((gemm_kn_kernel<1, 1, 3>), grid, dim3(256), 0, s, gg); break;
+        }
+    } else {
+        switch (am) {
+        case 1: hipLaunchKernelGGL((gemm_kn_kernel<0, 0, 2>), grid, dim3(256), 0, s, gg); break;
+        case 2: hipLaunchKernelGGL((gemm_kn_kernel<1, 0, 2>), grid, dim3(256), 0, s, gg); break;
+        case 3: hipLaunchKernelGGL((gemm_kn_kernel<0, 1, 2>), grid, dim3(256), 0, s, gg); break;
+        default: hipLaunchKernelGGL((gemm_kn_kernel<1, 1, 2>), grid, dim3(256), 0, s, gg); break;
+        }
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || g.splits == 1)
