@@ -423,20 +423,33 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
 // reads zero there: kzero). Tiles, waves, MFMA blocks, splits and the store are those of gemm_kernel.
 constexpr int KN_TILE = 16 * 64;  // doubles per operand chunk (8 KiB)
 
-// source of the 16-byte vector that lane `lane` of wave load L (0..7) puts into an operand chunk
+// source of the 16-byte vector that lane `lane` of wave load L (0..7) puts into an operand chunk;
+// mfold > 0 (A of a folded batch): tile row x is row x % mfold of batch entry x / mfold
 template <int MK>
 __device__ inline const double *kn_src(const GemmOp &op, int64_t base, int x0, int xext, int k1, int k2c, int K2, int L,
-                                       int lane) {
+                                       int lane, int mfold) {
     if constexpr (MK) {  // [m][k] tile: wave load L covers rows 8 L .. 8 L + 7
         const int mm = 8 * L + (lane >> 3), p = lane & 7;
         const int lv = p ^ ((mm >> 1) & 7);
         const int64_t k2 = min((int64_t)k2c * 16 + 2 * lv, op.cext - 2);
-        const int m = min(x0 + mm, xext - 1);
+        int m = min(x0 + mm, xext - 1);
+        if (mfold) {
+            const int b = m / mfold;
+            base += b * op.sb;
+            m -= b * mfold;
+        }
         return op.base + base + (int64_t)m * op.sr + (int64_t)k1 * op.s1 + k2;
     } else {  // [k][x] tile: wave load L covers rows 2 L, 2 L + 1
         const int kk = 2 * L + (lane >> 5), p = lane & 31;
         const int lv = p ^ ((kk & 1) << 3);
-        const int64_t x = min((int64_t)x0 + 2 * lv, op.cext - 2);
+        int64_t x;
+        if (mfold) {  // mfold is even, so the two rows of a vector lie in one batch entry
+            const int xf = min(x0 + 2 * lv, xext - 2), b = xf / mfold;
+            base += b * op.sb;
+            x = xf - b * mfold;
+        } else {
+            x = min((int64_t)x0 + 2 * lv, op.cext - 2);
+        }
         const int k2 = op.kzero ? k2c * 16 + kk : min(k2c * 16 + kk, K2 - 1);
         return op.base + base + (int64_t)k1 * op.s1 + (int64_t)k2 * op.s2 + x;
     }
@@ -493,8 +506,8 @@ __global__ __launch_bounds__(256) void gemm_kn_kernel(GemmGroup gg) {
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             const int L = 2 * wave + h;
-            const double *sa = kn_src<MK>(sg.A, (int64_t)beta * sg.A.sb, m0, g.M, k1, k2c, sg.K2, L, lane);
-            const double *sb = kn_src<0>(sg.B, (int64_t)beta * sg.B.sb, n0, g.N, k1, k2c, sg.K2, L, lane);
+            const double *sa = kn_src<MK>(sg.A, (int64_t)beta * sg.A.sb, m0, g.M, k1, k2c, sg.K2, L, lane, g.mfold);
+            const double *sb = kn_src<0>(sg.B, (int64_t)beta * sg.B.sb, n0, g.N, k1, k2c, sg.K2, L, lane, 0);
             __builtin_amdgcn_global_load_lds((const void *)sa, (lds_void *)(As + L * 128), 16, 0, 0);
             __builtin_amdgcn_global_load_lds((const void *)sb, (lds_void *)(Bs + L * 128), 16, 0, 0);
         }
@@ -558,7 +571,12 @@ __global__ __launch_bounds__(256) void gemm_kn_kernel(GemmGroup gg) {
                 const int m = m0 + wm * 32 + x * 16 + (lane >> 4) + 4 * r;
                 if (m < g.M && n < g.N) {
                     if (g.splits == 1) {
-                        double *cp = g.C + (int64_t)beta * g.sCb + (int64_t)m * g.sCm + (int64_t)n * g.sCn;
+                        int64_t row = (int64_t)beta * g.sCb + (int64_t)m * g.sCm;
+                        if (g.mfold) {  // row m of the folded batch: row m % mfold of entry m / mfold
+                            const int bq = m / g.mfold;
+                            row = (int64_t)bq * g.sCb + (int64_t)(m - bq * g.mfold) * g.sCm;
+                        }
+                        double *cp = g.C + row + (int64_t)n * g.sCn;
                         *cp += g.alpha * acc[x][y][r];
                     } else {
                         g.scratch[((int64_t)split * g.M + m) * g.N + n] = acc[x][y][r];
@@ -1070,7 +1088,27 @@ hipError_t launch_gemm(const GemmArgs &g, int batch, hipStream_t s) {
     return hipGetLastError();
 }
 
-static hipError_t launch_kn_group(const GemmGroup &gg, int am, int batch, hipStream_t s) {
+static hipError_t launch_kn_group(const GemmGroup &gg0, int am, int batch, hipStream_t s) {
+    // A batch whose B operands do not depend on the batch index (the dX and dTM back-contractions) is
+    // folded into the rows: M' = M x batch rows of 64-row tiles leave one partly filled tile instead of
+    // one per batch entry (at v = 200 the fourth tile of every entry holds 8 rows). Needs M even (the
+    // two rows of a 16-byte A vector then share a batch entry) and one split.
+    GemmGroup gg = gg0;
+    const char *ef = std::getenv("SCESS_KN_FOLD");  // tuning override: 0 keeps one tile row set per entry
+    bool fold = batch > 1 && !(ef && std::atoi(ef) == 0);
+    for (int p = 0; p < gg.n && fold; p++) {
+        const GemmArgs &q = gg.g[p];
+        fold = q.splits == 1 && q.M % 2 == 0 && (int64_t)q.M * batch < ((int64_t)1 << 30);
+        for (int h = 0; h < q.nseg && fold; h++)
+            fold = q.seg[h].B.sb == 0;
+    }
+    if (fold) {
+        for (int p = 0; p < gg.n; p++) {
+            gg.g[p].mfold = gg.g[p].M;
+            gg.g[p].M *= batch;
+        }
+        batch = 1;
+    }
     const GemmArgs &g = gg.g[0];
     const int ntiles = ((g.M + GT - 1) / GT) * ((g.N + GT - 1) / GT);
     const dim3 grid((unsigned)ntiles, (unsigned)batch, (unsigned)(g.splits * gg.n));

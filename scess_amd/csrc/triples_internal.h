@@ -170,6 +170,9 @@ struct GemmArgs {
     // into hi + lo bf16 for PB3) in registers, each 16-deep K chunk is summed on the MFMA in FP32
     // and added to FP64 accumulators. Other layouts always use FP64.
     int prec;
+    // Set by launch_gemm_group only (gemm_kn_kernel): > 0 when the batch is folded into the rows, M
+    // then being mfold x batch and row m' of A and C row m' % mfold of batch entry m' / mfold.
+    int mfold;
 };
 hipError_t launch_gemm(const GemmArgs &g, int batch, hipStream_t s);
 // Up to kMaxGemmGroup GEMMs of one shape (M, N, batch, splits) writing disjoint outputs, launched
