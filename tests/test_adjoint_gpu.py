@@ -44,7 +44,9 @@ def ctx():
 @pytest.mark.parametrize("o,v,seed", [(2, 3, 1), (3, 5, 2), (4, 9, 3), (5, 17, 4), (3, 70, 5), (6, 3, 6),
                                       (34, 7, 7)])
 def test_backward_matches_oracle(ctx, o, v, seed):
-    """(3, 70): v past one 64-wide tile of the back-contraction GEMMs. (6, 3) and (34, 7): o > v, where
+    """(3, 70): v past one 64-wide tile of the back-contraction GEMMs; v even, so the batched ones fold
+    the batch into their rows with tiles that span two batch entries (odd v keep one tile set per
+    entry). (6, 3) and (34, 7): o > v, where
     the (ooov) gradient (o^3 v entries) is the largest array the backward stages for download; (34, 7)
     also has o past two 16-wide occupied chunks and N = o past 32 in the dRn and dTM back-contractions."""
     _say(f"case backward o={o} v={v}")
