@@ -39,9 +39,9 @@ DTYPE = {"fp64": "f64", "fp32": "f32 (FP32 MFMA, FP32 accumulation; FP64 epilogu
 BAR_EH = 1.0e-6
 
 
-# the W-build kernel of each mode (template arguments: FP64 two fixed indices per work-group: stages;
+# the W-build kernel of each mode (template arguments: FP64 fixed indices per work-group, stages;
 # others: precision, MFMA blocks per wave edge, stages; int8: digits, fixed indices per work-group, stages)
-W_KERNEL = {"fp64": "wbuild_f2_kernel<2>", "fp32": "wbuild_dma_kernel<1, 2, 2>",
+W_KERNEL = {"fp64": "wbuild_fx_kernel<4, 2>", "fp32": "wbuild_dma_kernel<1, 2, 2>",
             "bf16x3": "wbuild_dma_kernel<2, 4, 2>", "ozaki": "wbuild_i8f_kernel<{S}, {F}, 2>"}
 
 

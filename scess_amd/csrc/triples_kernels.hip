@@ -718,30 +718,33 @@ __global__ __launch_bounds__(256) void wbuild_dma_kernel(WBuildArgs a) {
 // Whether the fixed index of segment s (get_seg numbering) sits on operand A (else on B).
 __device__ inline bool seg_fx_on_a(bool fam1, int s) { return fam1 && (s == 0 || s == 2 || s == 6 || s == 7); }
 
-// ------------------------------- W build, FP64, two fixed indices per work-group
-// wbuild_dma_kernel<P64, 2, STAGES> with two outputs per work-group: the 64 x 64 tile (tr, tc) of W
-// for the fixed indices fx0 = 2 g and fx0 + 1. In every segment exactly one operand depends on the
-// fixed index (seg_fx_on_a), so a stage holds the other operand's tile once (tile 0) and the
-// fixed-index operand's tile of each fixed index (tiles 1, 2): 24 KiB per stage where two work-groups
-// of wbuild_dma_kernel load 32 KiB, and 32 MFMAs per wave between barriers instead of 16. Wave w
-// computes fixed index w >> 1, columns 32 (w & 1) .. + 31 of the tile, all 64 rows (eight 16 x 16
-// blocks). The fragments and the order of the MFMAs into each accumulator are those of
-// wbuild_dma_kernel, so W is bitwise the same.
-template <int STAGES>
-__global__ __launch_bounds__(256) void wbuild_f2_kernel(WBuildArgs a) {
+// ------------------------------- W build, FP64, F fixed indices per work-group
+// wbuild_dma_kernel<P64, 2, STAGES> with F (2 or 4) outputs per work-group: the 64 x 64 tile (tr, tc)
+// of W for the fixed indices fx0 = F g .. fx0 + F - 1. In every segment exactly one operand depends on
+// the fixed index (seg_fx_on_a), so a stage holds the other operand's tile once (tile 0) and the
+// fixed-index operand's tile of each fixed index (tiles 1 .. F): F + 1 tiles of 8 KiB for F outputs,
+// where F one-index work-groups load 2F. 2F waves; wave w computes fixed index w >> 1, columns
+// 32 (w & 1) .. + 31 of the tile, all 64 rows (eight 16 x 16 blocks, 32 MFMAs per barrier against 16).
+// The fragments and the order of the MFMAs into each accumulator are those of wbuild_dma_kernel, so W
+// is bitwise the same. F = 4: 512 threads, 80 KiB of LDS and at most 128 registers (four waves per
+// SIMD), so two work-groups fill a CU.
+template <int F, int STAGES>
+__global__ __launch_bounds__(64 * 2 * F) __attribute__((amdgpu_waves_per_eu(4))) void wbuild_fx_kernel(WBuildArgs a) {
     constexpr int RB = 128, VPR = 8;  // 16 doubles per row chunk, as eight 16-byte vectors
     constexpr int OPB = 64 * RB;      // bytes of one operand tile chunk (8 KiB)
-    constexpr int NLW = 3 * OPB / 1024 / 4;  // 1 KiB wave loads per wave per stage (6)
+    constexpr int NW = 2 * F;                        // waves
+    constexpr int NLW = (F + 1) * OPB / 1024 / NW;  // 1 KiB wave loads per wave per stage
+    static_assert((F + 1) * OPB / 1024 % NW == 0, "loads split evenly over the waves");
     static_assert(STAGES == 2 || STAGES == 3, "stages");
-    __shared__ __attribute__((aligned(1024))) char lds[STAGES][3 * OPB];
+    __shared__ __attribute__((aligned(1024))) char lds[STAGES][(F + 1) * OPB];
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int wf = wave >> 1, wp = wave & 1;  // fixed index (0, 1) and column half of this wave
+    const int wf = wave >> 1, wp = wave & 1;  // fixed index (0 .. F-1) and column half of this wave
     const int lr = lane & 15, lq = lane >> 4;
 
     const int nt2 = a.nt * a.nt;
-    const int nfg = (a.v + 1) >> 1;  // fixed-index pairs per family
+    const int nfg = (a.v + F - 1) / F;  // fixed-index groups per family
     const int gx = gridDim.x;
     const int lin = xcd_remap(blockIdx.y * gx + blockIdx.x, gx * gridDim.y);
     const int bat = lin / gx;
@@ -756,9 +759,9 @@ __global__ __launch_bounds__(256) void wbuild_f2_kernel(WBuildArgs a) {
     const int64_t i = q[0], j = q[1], k = q[2];
     const int v = a.v;
     const int r0 = tr * 64, c0 = tc * 64;
-    const int64_t fx0 = 2 * (int64_t)fg;
-    const int64_t fx1 = min(fx0 + 1, (int64_t)v - 1);  // with v odd the last pair's second index loads v - 1
-    const bool live = fx0 + wf < v;                    // and its waves compute and store nothing
+    const int64_t fx0 = (int64_t)F * fg;
+    const int64_t fx1 = min(fx0 + 1, (int64_t)v - 1);  // indices past v load index v - 1
+    const bool live = fx0 + wf < v;                    // and their waves compute and store nothing
 
     const bool deg = a.halve && i == j;
     if (deg && !fam1)
@@ -781,12 +784,14 @@ __global__ __launch_bounds__(256) void wbuild_f2_kernel(WBuildArgs a) {
         for (int y = 0; y < 2; y++)
             acc[x][y] = d4{};
 
-    // wave load L = wave * NLW + u fills LDS KiB L of a stage: tile L / 8, rows 8 (L % 8) .. + 7; the
-    // lane's row in its tile and the byte offset of the logical vector it fetches (fixed per lane)
+    // wave load L = u * NW + wave fills LDS KiB L of a stage: tile L / 8 (a constant of u, since NW is 4
+    // or 8), rows 8 (L % 8) .. + 7; the lane's row in its tile and the byte offset of the logical vector
+    // it fetches (fixed per lane)
+    constexpr auto tile_of = [](int u) { return u * NW / 8; };
     int lrow[NLW], lvoff[NLW];
 #pragma unroll
     for (int u = 0; u < NLW; u++) {
-        const int L = wave * NLW + u;
+        const int L = u * NW + wave;
         const int r = (L & 7) * 8 + (lane >> 3), p = lane & 7;
         lrow[u] = r;
         lvoff[u] = (p ^ lds_swz<VPR>(r)) * 16;
@@ -799,17 +804,18 @@ __global__ __launch_bounds__(256) void wbuild_f2_kernel(WBuildArgs a) {
     bool fa = false;
     auto set_seg = [&](int sgi) {
         const Seg s0 = get_seg(a, fam1, sgi, i, j, k, fx0);
-        const Seg s1 = get_seg(a, fam1, sgi, i, j, k, fx1);
+        const Seg s1 = get_seg(a, fam1, sgi, i, j, k, fx1);  // address arithmetic only: the step per index
         fa = seg_fx_on_a(fam1, sgi);
         nch = s0.nch;
+        const int64_t dfx = fa ? s1.A - s0.A : s1.B - s0.B;  // bytes per fixed index (0 if fx1 == fx0)
 #pragma unroll
         for (int u = 0; u < NLW; u++) {
-            const int tl = (wave * NLW + u) >> 3;  // wave-uniform
-            const Seg &sx = tl == 2 ? s1 : s0;
-            // tile 0: the operand without the fixed index; tiles 1, 2: the one with it
+            const int tl = tile_of(u);
+            // tile 0: the operand without the fixed index; tile 1 + t: the one with it, index fx0 + t
             const bool use_a = (tl == 0) != fa;
-            const char *base = use_a ? sx.A : sx.B;
-            const int64_t ld = use_a ? sx.lda : sx.ldb;
+            const int64_t t = tl == 0 ? 0 : min(fx0 + tl - 1, (int64_t)v - 1) - fx0;
+            const char *base = (use_a ? s0.A : s0.B) + t * dfx;
+            const int64_t ld = use_a ? s0.lda : s0.ldb;
             const int row = min((use_a ? r0 : c0) + lrow[u], v - 1);
             pl[u] = base + row * ld + lvoff[u];
         }
@@ -822,7 +828,7 @@ __global__ __launch_bounds__(256) void wbuild_f2_kernel(WBuildArgs a) {
 #pragma unroll
         for (int u = 0; u < NLW; u++)
             __builtin_amdgcn_global_load_lds((const void *)(pl[u] + off),
-                                             (lds_void *)(base + (wave * NLW + u) * 1024), 16, 0, 0);
+                                             (lds_void *)(base + (u * NW + wave) * 1024), 16, 0, 0);
         famask = fa ? famask | (1 << st) : famask & ~(1 << st);
         if (++kc == nch) {
             kc = 0;
@@ -1159,16 +1165,21 @@ template <int PREC>
 static hipError_t launch_wbuild_p(int blk, bool dma, const WBuildArgs &a, int nbatch, hipStream_t s) {
     const dim3 grid((unsigned)(2 * a.ntile1), (unsigned)nbatch);
     if constexpr (PREC == P64) {
-        // FP64 64 x 64 tiles: two fixed indices per work-group (SCESS_WBUILD_F2=0: one, tuning override)
+        // FP64 64 x 64 tiles: four fixed indices per work-group (SCESS_WBUILD_F2=0: one, tuning override)
         const char *e = std::getenv("SCESS_WBUILD_F2");
         if (dma && blk == 2 && !(e && std::atoi(e) == 0)) {
-            const int nfg = (a.v + 1) / 2;
-            const dim3 g2((unsigned)(2 * nfg * a.nt * a.nt), (unsigned)nbatch);
-            const char *es = std::getenv("SCESS_WBUILD_STAGES");  // tuning override: 2 or 3
-            if (es && std::atoi(es) == 3)
-                hipLaunchKernelGGL((wbuild_f2_kernel<3>), g2, dim3(256), 0, s, a);
-            else
-                hipLaunchKernelGGL((wbuild_f2_kernel<2>), g2, dim3(256), 0, s, a);
+            const dim3 g2((unsigned)(2 * ((a.v + 1) / 2) * a.nt * a.nt), (unsigned)nbatch);
+            const char *es = std::getenv("SCESS_WBUILD_STAGES");  // tuning override: 3 (two indices)
+            const char *ef = std::getenv("SCESS_WBUILD_FX");      // tuning override: 2 fixed indices
+            if ((ef && std::atoi(ef) == 2) || (es && std::atoi(es) == 3)) {
+                if (es && std::atoi(es) == 3)
+                    hipLaunchKernelGGL((wbuild_fx_kernel<2, 3>), g2, dim3(256), 0, s, a);
+                else
+                    hipLaunchKernelGGL((wbuild_fx_kernel<2, 2>), g2, dim3(256), 0, s, a);
+            } else {
+                const dim3 g4((unsigned)(2 * ((a.v + 3) / 4) * a.nt * a.nt), (unsigned)nbatch);
+                hipLaunchKernelGGL((wbuild_fx_kernel<4, 2>), g4, dim3(512), 0, s, a);
+            }
             return hipGetLastError();
         }
     }
