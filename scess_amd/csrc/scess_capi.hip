@@ -1174,8 +1174,8 @@ int scess_t_backward(scess_ctx *c, int prec, double dE, scess_t_grad *grad) {
         if (he == hipSuccess)
             he = launch_gemm_group(grp, (int)v, c->stream);
         if (he == hipSuccess)
-            he = launch_vback(GV, U, t.t1, occ[0], occ[1], occ[2], (int)o, (int)v, (int)vR, dU, dtv, dS, dt1,
-                              c->stream);
+            he = launch_vback(GV, U, t.t1, occ[0], occ[1], occ[2], (int)o, (int)v, (int)vR, dU, dtv, scr, dS, dt1,
+                              c->stream);  // scr (the split-K scratch, >= v^2, free again here) holds H
         // the last triple of a batch releases its W set
         if (he == hipSuccess && (slot == WB - 1 || w == w_end - 1))
             he = hipEventRecord(evDone[set], c->stream);

@@ -1199,11 +1199,13 @@ template <typename F> __device__ inline void nsum8(int n, F f, double &s, double
         nadd_a(s, c, f(z));
 }
 
-__global__ __launch_bounds__(256) void vback_du_kernel(const double *GV, const double *t1, int i, int j, int k, int v,
-                                                        int vR, double *dU) {
+// dU (w = 0, 1, 2) and H[a][c] = sum_b G_V[a][b][c] U0[a][b] (w = 3, into H), one thread per output
+// element; the threads of a wave take consecutive y, so every G_V read but dU0's is coalesced
+__global__ __launch_bounds__(256) void vback_du_kernel(const double *GV, const double *t1, const double *U, int i,
+                                                        int j, int k, int v, int vR, double *dU, double *H) {
     const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t vv = (int64_t)v * v;
-    if (idx >= 3 * vv)
+    if (idx >= 4 * vv)
         return;
     const int w = (int)(idx / vv);
     const int64_t xy = idx - w * vv;
@@ -1217,22 +1219,28 @@ __global__ __launch_bounds__(256) void vback_du_kernel(const double *GV, const d
     } else if (w == 1) {  // dU1[x=a][y=c]
         const double *t = t1 + (int64_t)j * vR;
         nsum8(v, [&](int z) { return GV[x * R2 + (int64_t)z * vR + y] * t[z]; }, s, c);
-    } else {  // dU2[x=b][y=c]
+    } else if (w == 2) {  // dU2[x=b][y=c]
         const double *t = t1 + (int64_t)i * vR;
         nsum8(v, [&](int z) { return GV[z * R2 + (int64_t)x * vR + y] * t[z]; }, s, c);
+    } else {  // H[x=a][y=c]
+        const double *u0 = U + (int64_t)x * vR;
+        nsum8(v, [&](int z) { return GV[x * R2 + (int64_t)z * vR + y] * u0[z]; }, s, c);
+        H[xy] = s + c;
+        return;
     }
     dU[idx] = s + c;
 }
 
-__global__ __launch_bounds__(256) void vback_dt_kernel(const double *GV, const double *U, int v, int vR, double *dt) {
+__global__ __launch_bounds__(256) void vback_dt_kernel(const double *GV, const double *U, const double *H, int v, int vR,
+                                                        double *dt) {
     __shared__ double rs[256], rc[256];
     const int w = blockIdx.x / v, x = blockIdx.x - w * v;
     const int64_t R2 = (int64_t)vR * vR, vr = (int64_t)v * vR;
-    const int64_t n = (int64_t)v * v;
+    const int64_t n = w == 0 ? v : (int64_t)v * v;
     double s = 0.0, c = 0.0;
-    // terms t = p v + q: dt_k[x=c] = sum_ab G[a][b][c] U0[a][b] (w = 0), dt_j[x=b] = sum_ac G[a][b][c]
-    // U1[a][c] (w = 1), dt_i[x=a] = sum_bc G[a][b][c] U2[b][c] (w = 2); this thread takes t = tid, tid + 256,
-    // ... in order, eight loads in flight, (p, q) advanced without divisions
+    // terms t = p v + q: dt_j[x=b] = sum_ac G[a][b][c] U1[a][c] (w = 1), dt_i[x=a] = sum_bc G[a][b][c]
+    // U2[b][c] (w = 2); dt_k[x=c] = sum_a H[a][c] (w = 0, terms t = a, H from vback_du_kernel). This thread
+    // takes t = tid, tid + 256, ... in order, eight loads in flight, (p, q) advanced without divisions
     const double *Uw = U + (int64_t)w * vr;
     int p = (int)(threadIdx.x / v), q = (int)(threadIdx.x % v);
     for (int64_t t0 = threadIdx.x; t0 < n; t0 += 256 * 8) {
@@ -1241,10 +1249,11 @@ __global__ __launch_bounds__(256) void vback_dt_kernel(const double *GV, const d
         for (int u = 0; u < 8; u++) {
             double term = 0.0;  // past the end: adds nothing
             if (t0 + 256 * u < n) {
-                const double g = w == 0   ? GV[p * R2 + (int64_t)q * vR + x]
-                                 : w == 1 ? GV[p * R2 + (int64_t)x * vR + q]
-                                          : GV[x * R2 + (int64_t)p * vR + q];
-                term = g * Uw[(int64_t)p * vR + q];
+                if (w == 0)
+                    term = H[(t0 + 256 * u) * v + x];
+                else
+                    term = (w == 1 ? GV[p * R2 + (int64_t)x * vR + q] : GV[x * R2 + (int64_t)p * vR + q]) *
+                           Uw[(int64_t)p * vR + q];
             }
             xs[u] = term;
             q += 256;
@@ -1273,40 +1282,39 @@ __global__ __launch_bounds__(256) void vback_dt_kernel(const double *GV, const d
         dt[blockIdx.x] = rs[0] + rc[0];
 }
 
-// dst[x][y] += alpha * src[trans ? y : x][trans ? x : y] over v x v (dst pitch ldd, src pitch v)
-__global__ __launch_bounds__(256) void axpy2_kernel(double *dst, int64_t ldd, const double *src, int v, int trans,
-                                                     double alpha) {
+// dS[p][q][x][y] = d/d(px|qy): the six updates dS[pq_u] += 1/2 dU[u / 2] (transposed for odd u), in u
+// order, and the three dt1 row updates (k from dt[0..v), j from dt[v..2v), i from dt[2v..3v)), in order.
+// Targets may coincide (two equal occupied indices); each thread owns one (x, y) of every target, so the
+// updates of one element stay in that order and no two threads touch the same element.
+__global__ __launch_bounds__(256) void vback_acc_kernel(double *dS, double *dt1, const double *dU, const double *dt,
+                                                         int i, int j, int k, int o, int v, int vR) {
     const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (idx >= (int64_t)v * v)
-        return;
-    const int x = (int)(idx / v), y = (int)(idx - (int64_t)x * v);
-    dst[(int64_t)x * ldd + y] += alpha * (trans ? src[(int64_t)y * v + x] : src[idx]);
-}
-
-__global__ __launch_bounds__(256) void vadd_kernel(double *dst, const double *src, int n) {
-    const int idx = blockIdx.x * 256 + threadIdx.x;
-    if (idx < n)
-        dst[idx] += src[idx];
+    const int64_t vv = (int64_t)v * v;
+    if (idx < vv) {
+        const int x = (int)(idx / v), y = (int)(idx - (int64_t)x * v);
+        const int pq[6][2] = {{i, j}, {j, i}, {i, k}, {k, i}, {j, k}, {k, j}};
+#pragma unroll
+        for (int u = 0; u < 6; u++) {
+            const double *src = dU + (u / 2) * vv;
+            dS[((int64_t)pq[u][0] * o + pq[u][1]) * vv + idx] += 0.5 * ((u & 1) ? src[(int64_t)y * v + x] : src[idx]);
+        }
+    }
+    if (idx < v) {
+        const int rows[3] = {k, j, i};
+#pragma unroll
+        for (int u = 0; u < 3; u++)
+            dt1[(int64_t)rows[u] * vR + idx] += dt[(int64_t)u * v + idx];
+    }
 }
 
 hipError_t launch_vback(const double *GV, const double *U, const double *t1, int i, int j, int k, int o, int v, int vR,
-                        double *dU, double *dt, double *dS, double *dt1, hipStream_t s) {
+                        double *dU, double *dt, double *H, double *dS, double *dt1, hipStream_t s) {
     const int64_t vv = (int64_t)v * v;
-    hipLaunchKernelGGL(vback_du_kernel, dim3((unsigned)((3 * vv + 255) / 256)), dim3(256), 0, s, GV, t1, i, j, k, v, vR,
-                       dU);
-    hipLaunchKernelGGL(vback_dt_kernel, dim3((unsigned)(3 * v)), dim3(256), 0, s, GV, U, v, vR, dt);
-    // dS[p][q][x][y] = d/d(px|qy); six ordered updates, each its own launch (targets may coincide)
-    const dim3 gb((unsigned)((vv + 255) / 256));
-    const int pq[6][2] = {{i, j}, {j, i}, {i, k}, {k, i}, {j, k}, {k, j}};
-    for (int u = 0; u < 6; u++) {
-        double *dst = dS + ((int64_t)pq[u][0] * o + pq[u][1]) * vv;
-        hipLaunchKernelGGL(axpy2_kernel, gb, dim3(256), 0, s, dst, (int64_t)v, dU + (u / 2) * vv, v, u & 1, 0.5);
-    }
-    // dt1 rows (pitch vR): k from dt[0..v), j from dt[v..2v), i from dt[2v..3v); rows may coincide
-    const int rows[3] = {k, j, i};
-    for (int u = 0; u < 3; u++)
-        hipLaunchKernelGGL(vadd_kernel, dim3((unsigned)((v + 255) / 256)), dim3(256), 0, s, dt1 + (int64_t)rows[u] * vR,
-                           dt + (int64_t)u * v, v);
+    hipLaunchKernelGGL(vback_du_kernel, dim3((unsigned)((4 * vv + 255) / 256)), dim3(256), 0, s, GV, t1, U, i, j, k, v,
+                       vR, dU, H);
+    hipLaunchKernelGGL(vback_dt_kernel, dim3((unsigned)(3 * v)), dim3(256), 0, s, GV, U, H, v, vR, dt);
+    hipLaunchKernelGGL(vback_acc_kernel, dim3((unsigned)((std::max<int64_t>(vv, v) + 255) / 256)), dim3(256), 0, s, dS,
+                       dt1, dU, dt, i, j, k, o, v, vR);
     return hipGetLastError();
 }
 

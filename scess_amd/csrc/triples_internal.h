@@ -187,9 +187,10 @@ struct GemmGroup {
 hipError_t launch_gemm_group(const GemmGroup &gg, int batch, hipStream_t s);
 // G_W, G_V ([vR]^3 row-major, FP64) of the single occupied triple a.ijk; W in a.W1/W2, V pieces in a.U
 hipError_t launch_adjoint_g(int prec, const EpiArgs &a, double wgt, double *GW, double *GV, hipStream_t s);
-// V backward of one occupied triple into dS [o][o][v][v] and dt1 [o][vR]; dU (3 v^2) and dt (3 v) are scratch
+// V backward of one occupied triple into dS [o][o][v][v] and dt1 [o][vR]; dU (3 v^2), dt (3 v) and H (v^2)
+// are scratch
 hipError_t launch_vback(const double *GV, const double *U, const double *t1, int i, int j, int k, int o, int v, int vR,
-                        double *dU, double *dt, double *dS, double *dt1, hipStream_t s);
+                        double *dU, double *dt, double *H, double *dS, double *dt1, hipStream_t s);
 // ColMajor gradients: mode 0 ovvv <- dX, 1 ovoo <- -dRn, 2 ovov <- dS, 3 t2 <- dT + dTM, 4 t1 <- dt1
 hipError_t launch_unpack(int mode, const double *A, const double *B, double *out, int o, int v, int64_t vK, int64_t oK,
                          int64_t vR, int64_t n, hipStream_t s);
