@@ -760,8 +760,9 @@ __global__ __launch_bounds__(64 * 2 * F) __attribute__((amdgpu_waves_per_eu(4)))
     const int v = a.v;
     const int r0 = tr * 64, c0 = tc * 64;
     const int64_t fx0 = (int64_t)F * fg;
-    const int64_t fx1 = min(fx0 + 1, (int64_t)v - 1);  // indices past v load index v - 1
-    const bool live = fx0 + wf < v;                    // and their waves compute and store nothing
+    const int64_t fx1 = min(fx0 + 1, (int64_t)v - 1);  // gives the address step per fixed index
+    // fixed indices past v load the operand rows of index v - 1; their waves compute and store nothing
+    const bool live = fx0 + wf < v;
 
     const bool deg = a.halve && i == j;
     if (deg && !fam1)
@@ -867,7 +868,7 @@ __global__ __launch_bounds__(64 * 2 * F) __attribute__((amdgpu_waves_per_eu(4)))
         if (st < total)
             issue(st);
     for (int c = 0; c < total; c++) {
-        // this wave's loads of chunk c have landed once at most `ahead` later chunks are outstanding
+        // this wave's loads of chunk c have landed once at most the next chunk's are outstanding
         if (STAGES > 2 && c + 1 < total)
             wait_vm<(STAGES > 2 ? NLW : 0)>();
         else
