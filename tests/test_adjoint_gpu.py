@@ -90,6 +90,24 @@ def test_backward_scales_with_dE_and_is_deterministic(ctx):
         assert np.abs(g3[k] + 2.5 * g1[k]).max() <= 1e-13 * np.abs(g1[k]).max(), k
 
 
+@pytest.mark.parametrize("o,v,seed", [(3, 70, 8), (4, 40, 9)])
+def test_backward_folded_batches_match_unfolded(ctx, monkeypatch, o, v, seed):
+    """The batched back-contractions (dX, dTM) with the batch folded into the rows (default for even v)
+    against one tile set per batch entry (SCESS_KN_FOLD=0, read at each launch): the same MFMAs reach
+    every output element in the same order, only the element's row within its 16 x 16 block differs."""
+    _say(f"case backward fold o={o} v={v}")
+    inp = make_triples_inputs(o, v, seed)
+    ctx.set_triples_inputs(inp, ("fp64",))
+    monkeypatch.delenv("SCESS_KN_FOLD", raising=False)
+    g1 = ctx.triples_backward("fp64")
+    monkeypatch.setenv("SCESS_KN_FOLD", "0")
+    g0 = ctx.triples_backward("fp64")
+    monkeypatch.delenv("SCESS_KN_FOLD")
+    for k in NAMES:
+        scale = max(np.abs(g0[k]).max(), 1e-300)
+        assert np.abs(g1[k] - g0[k]).max() <= 1e-14 * scale, k
+
+
 def local_fd(ctx, inp, name, flat_index, h, precision="fp64"):
     """Central difference of E(T) in one entry from local partial differences."""
     arr = getattr(inp, name)

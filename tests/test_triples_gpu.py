@@ -90,6 +90,29 @@ def test_deterministic(ctx):
         assert E1 == E2, p
 
 
+@pytest.mark.parametrize("o,v,seed", [(5, 70, 15), (4, 67, 16), (6, 130, 17)])
+def test_fp64_wbuild_variants_bitwise_equal(ctx, monkeypatch, o, v, seed):
+    """The FP64 W-build with four fixed indices per work-group (default), two (SCESS_WBUILD_FX=2) and
+    one (SCESS_WBUILD_F2=0) feeds every accumulator the same MFMAs in the same order, so E(T) and every
+    F_ijk are bitwise equal; v = 67 leaves the last group of four with one live index, v = 130 with two.
+    The variants are read at each launch."""
+    _say(f"case fp64 W-build variants o={o} v={v}")
+    inp = make_triples_inputs(o, v, seed)
+    ctx.set_triples_inputs(inp, ("fp64",))
+    monkeypatch.delenv("SCESS_WBUILD_FX", raising=False)
+    monkeypatch.delenv("SCESS_WBUILD_F2", raising=False)
+    E4, F4, _ = ctx.triples_forward("fp64", with_F=True)
+    monkeypatch.setenv("SCESS_WBUILD_FX", "2")
+    E2, F2, _ = ctx.triples_forward("fp64", with_F=True)
+    monkeypatch.delenv("SCESS_WBUILD_FX")
+    monkeypatch.setenv("SCESS_WBUILD_F2", "0")
+    E1, F1, _ = ctx.triples_forward("fp64", with_F=True)
+    monkeypatch.delenv("SCESS_WBUILD_F2")
+    assert E4 == E2 == E1, (E4, E2, E1)
+    assert np.array_equal(F4, F2) and np.array_equal(F4, F1)
+    assert _rel(E4, oracle.t_abc(inp)) < 1e-12
+
+
 @pytest.mark.parametrize("o,v,seed", [(4, 20, 11), (8, 64, 12), (10, 100, 13)])
 def test_mixed_precision_close_to_fp64(ctx, o, v, seed):
     _say(f"case mixed o={o} v={v}")
