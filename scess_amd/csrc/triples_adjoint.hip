@@ -658,24 +658,29 @@ __device__ inline float bf16_value(short h) { return __uint_as_float(((uint32_t)
 // DMA: operand chunks go from global memory straight into LDS (global_load_lds_dwordx4, each lane fetching
 // the logical vector its swizzled LDS slot holds), as in the W-build; vectors reaching past K2 load the
 // row's first vector instead and are rewritten (value or zero) after the chunk has landed.
-template <int BLK, bool VEC, int PREC, bool DMA>
+// WIDE = 2 (direct-to-LDS only): tiles of BM rows by 2 BM columns, each wave BLK x 2 BLK blocks, so a
+// wave issues twice the MFMAs per chunk for 1.5x the staged bytes.
+template <int BLK, bool VEC, int PREC, bool DMA, int WIDE>
 __global__ __launch_bounds__(256) void gemm_kc_kernel(GemmArgs g) {
-    constexpr int BM = 2 * BLK * 16;   // tile rows (and columns)
+    constexpr int BM = 2 * BLK * 16;   // tile rows
+    constexpr int BN = BM * WIDE;      // tile columns
+    constexpr int BLKN = BLK * WIDE;   // MFMA blocks per wave along n
     constexpr int RB = 128;            // bytes of one 16-double K chunk of a row
-    constexpr int NV = BM * 8 / 256;   // 16-byte vectors per thread per operand per chunk
+    constexpr int NV = BM * 8 / 256;   // 16-byte vectors per thread per operand per chunk (square tiles)
     static_assert(!DMA || (VEC && BM * RB % 4096 == 0), "direct-to-LDS loads need 16-byte vectors, whole KiB");
-    __shared__ __attribute__((aligned(1024))) char lds[2][2 * BM * RB];
+    static_assert(WIDE == 1 || (WIDE == 2 && DMA), "wide tiles use the direct-to-LDS loads");
+    __shared__ __attribute__((aligned(1024))) char lds[2][(BM + BN) * RB];
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = wave >> 1, wn = wave & 1;
     const int lr = lane & 15, lq = lane >> 4;
-    const int ntn = (g.N + BM - 1) / BM;
+    const int ntn = (g.N + BN - 1) / BN;
     // tiles of one (batch, split) on one XCD, so the tiles sharing an operand panel share its L2
     const int nx = gridDim.x, ny = gridDim.y;
     const int lin = xcd_remap(blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z), nx * ny * gridDim.z);
     const int bx = lin % nx, byz = lin / nx;
     const int tm = bx / ntn, tn = bx - tm * ntn;
-    const int m0 = tm * BM, n0 = tn * BM;
+    const int m0 = tm * BM, n0 = tn * BN;
     const int beta = byz % ny, split = byz / ny;
 
     // segment values as scalars: a runtime-indexed local array would live in scratch memory
@@ -684,19 +689,20 @@ __global__ __launch_bounds__(256) void gemm_kc_kernel(GemmArgs g) {
     const int total = nch0 + nch1;
     const int c_begin = (int)((int64_t)total * split / g.splits), c_end = (int)((int64_t)total * (split + 1) / g.splits);
 
-    d4 acc[BLK][BLK];
+    d4 acc[BLK][BLKN];
 #pragma unroll
     for (int x = 0; x < BLK; x++)
 #pragma unroll
-        for (int y = 0; y < BLK; y++)
+        for (int y = 0; y < BLKN; y++)
             acc[x][y] = d4{0.0, 0.0, 0.0, 0.0};
     // MFMA blocks of this wave wholly past M or N are skipped (wave-uniform)
-    bool vm[BLK], vn[BLK];
+    bool vm[BLK], vn[BLKN];
 #pragma unroll
-    for (int x = 0; x < BLK; x++) {
+    for (int x = 0; x < BLK; x++)
         vm[x] = m0 + wm * BLK * 16 + x * 16 < g.M;
-        vn[x] = n0 + wn * BLK * 16 + x * 16 < g.N;
-    }
+#pragma unroll
+    for (int y = 0; y < BLKN; y++)
+        vn[y] = n0 + wn * BLKN * 16 + y * 16 < g.N;
 
     // thread t loads vectors q = t + 256 u (u < NV) of the BM x 8 vector tile: row q / 8, vector q % 8
     d2v ra[NV], rb[NV];
@@ -754,11 +760,15 @@ __global__ __launch_bounds__(256) void gemm_kc_kernel(GemmArgs g) {
         if constexpr (PREC == P64) {
 #pragma unroll
             for (int h = 0; h < 2; h++) {
-                d2v af[BLK], bf[BLK];
+                d2v af[BLK], bf[BLKN];
 #pragma unroll
                 for (int b = 0; b < BLK; b++) {
-                    const int rowa = wm * BLK * 16 + b * 16 + lr, rowb = wn * BLK * 16 + b * 16 + lr;
+                    const int rowa = wm * BLK * 16 + b * 16 + lr;
                     af[b] = *(const d2v *)(As + rowa * RB + (((2 * lq + h) ^ kswz(rowa)) << 4));
+                }
+#pragma unroll
+                for (int b = 0; b < BLKN; b++) {
+                    const int rowb = wn * BLKN * 16 + b * 16 + lr;
                     bf[b] = *(const d2v *)(Bs + rowb * RB + (((2 * lq + h) ^ kswz(rowb)) << 4));
                 }
 #pragma unroll
@@ -767,7 +777,7 @@ __global__ __launch_bounds__(256) void gemm_kc_kernel(GemmArgs g) {
                     for (int x = 0; x < BLK; x++)
                         if (vm[x])
 #pragma unroll
-                            for (int y = 0; y < BLK; y++)
+                            for (int y = 0; y < BLKN; y++)
                                 if (vn[y])
                                     acc[x][y] =
                                         __builtin_amdgcn_mfma_f64_16x16x4f64(af[x][t], bf[y][t], acc[x][y], 0, 0, 0);
@@ -776,19 +786,23 @@ __global__ __launch_bounds__(256) void gemm_kc_kernel(GemmArgs g) {
             // a 16-slot chunk holds 32 k of pre-split rows: vector 2 lq = the hi bf16 of k = 8 lq .. 8 lq + 7,
             // vector 2 lq + 1 their lo bf16 (the 16x16x32 operand layout); hi*hi + hi*lo + lo*hi in FP32 per
             // chunk, added to the FP64 accumulators
-            b8v ah[BLK], al[BLK], bh[BLK], bl[BLK];
+            b8v ah[BLK], al[BLK], bh[BLKN], bl[BLKN];
 #pragma unroll
             for (int b = 0; b < BLK; b++) {
-                const int rowa = wm * BLK * 16 + b * 16 + lr, rowb = wn * BLK * 16 + b * 16 + lr;
+                const int rowa = wm * BLK * 16 + b * 16 + lr;
                 ah[b] = *(const b8v *)(As + rowa * RB + (((2 * lq) ^ kswz(rowa)) << 4));
                 al[b] = *(const b8v *)(As + rowa * RB + (((2 * lq + 1) ^ kswz(rowa)) << 4));
+            }
+#pragma unroll
+            for (int b = 0; b < BLKN; b++) {
+                const int rowb = wn * BLKN * 16 + b * 16 + lr;
                 bh[b] = *(const b8v *)(Bs + rowb * RB + (((2 * lq) ^ kswz(rowb)) << 4));
                 bl[b] = *(const b8v *)(Bs + rowb * RB + (((2 * lq + 1) ^ kswz(rowb)) << 4));
             }
 #pragma unroll
             for (int x = 0; x < BLK; x++)
 #pragma unroll
-                for (int y = 0; y < BLK; y++) {
+                for (int y = 0; y < BLKN; y++) {
                     if (!vm[x] || !vn[y])
                         continue;
                     f4v c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[x], bh[y], f4v{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
@@ -800,22 +814,26 @@ __global__ __launch_bounds__(256) void gemm_kc_kernel(GemmArgs g) {
                 }
         } else {
             // the lane's 4 k values of each block row, rounded (FP32) or split (hi + lo BF16)
-            f4v a32[BLK], b32[BLK];
+            f4v a32[BLK], b32[BLKN];
 #pragma unroll
             for (int b = 0; b < BLK; b++) {
-                const int rowa = wm * BLK * 16 + b * 16 + lr, rowb = wn * BLK * 16 + b * 16 + lr;
+                const int rowa = wm * BLK * 16 + b * 16 + lr;
                 const d2v a0 = *(const d2v *)(As + rowa * RB + (((2 * lq) ^ kswz(rowa)) << 4));
                 const d2v a1 = *(const d2v *)(As + rowa * RB + (((2 * lq + 1) ^ kswz(rowa)) << 4));
+                a32[b] = f4v{(float)a0.x, (float)a0.y, (float)a1.x, (float)a1.y};
+            }
+#pragma unroll
+            for (int b = 0; b < BLKN; b++) {
+                const int rowb = wn * BLKN * 16 + b * 16 + lr;
                 const d2v b0 = *(const d2v *)(Bs + rowb * RB + (((2 * lq) ^ kswz(rowb)) << 4));
                 const d2v b1 = *(const d2v *)(Bs + rowb * RB + (((2 * lq + 1) ^ kswz(rowb)) << 4));
-                a32[b] = f4v{(float)a0.x, (float)a0.y, (float)a1.x, (float)a1.y};
                 b32[b] = f4v{(float)b0.x, (float)b0.y, (float)b1.x, (float)b1.y};
             }
-            f4v c32[BLK][BLK];
+            f4v c32[BLK][BLKN];
 #pragma unroll
             for (int x = 0; x < BLK; x++)
 #pragma unroll
-                for (int y = 0; y < BLK; y++)
+                for (int y = 0; y < BLKN; y++)
                     c32[x][y] = f4v{0.f, 0.f, 0.f, 0.f};
             if constexpr (PREC == P32) {
 #pragma unroll
@@ -824,25 +842,30 @@ __global__ __launch_bounds__(256) void gemm_kc_kernel(GemmArgs g) {
                     for (int x = 0; x < BLK; x++)
                         if (vm[x])
 #pragma unroll
-                            for (int y = 0; y < BLK; y++)
+                            for (int y = 0; y < BLKN; y++)
                                 if (vn[y])
                                     c32[x][y] =
                                         __builtin_amdgcn_mfma_f32_16x16x4f32(a32[x][t], b32[y][t], c32[x][y], 0, 0, 0);
             } else {
-                s4v ah[BLK], al[BLK], bh[BLK], bl[BLK];
+                s4v ah[BLK], al[BLK], bh[BLKN], bl[BLKN];
 #pragma unroll
                 for (int b = 0; b < BLK; b++)
 #pragma unroll
                     for (int t = 0; t < 4; t++) {
                         ah[b][t] = bf16_bits(a32[b][t]);
                         al[b][t] = bf16_bits(a32[b][t] - bf16_value(ah[b][t]));
+                    }
+#pragma unroll
+                for (int b = 0; b < BLKN; b++)
+#pragma unroll
+                    for (int t = 0; t < 4; t++) {
                         bh[b][t] = bf16_bits(b32[b][t]);
                         bl[b][t] = bf16_bits(b32[b][t] - bf16_value(bh[b][t]));
                     }
 #pragma unroll
                 for (int x = 0; x < BLK; x++)
 #pragma unroll
-                    for (int y = 0; y < BLK; y++) {
+                    for (int y = 0; y < BLKN; y++) {
                         if (!vm[x] || !vn[y])
                             continue;
                         c32[x][y] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al[x], bh[y], c32[x][y], 0, 0, 0);
@@ -853,7 +876,7 @@ __global__ __launch_bounds__(256) void gemm_kc_kernel(GemmArgs g) {
 #pragma unroll
             for (int x = 0; x < BLK; x++)
 #pragma unroll
-                for (int y = 0; y < BLK; y++)
+                for (int y = 0; y < BLKN; y++)
 #pragma unroll
                     for (int r = 0; r < 4; r++)
                         acc[x][y][r] += (double)c32[x][y][r];
@@ -862,14 +885,22 @@ __global__ __launch_bounds__(256) void gemm_kc_kernel(GemmArgs g) {
 
     int buf = 0;
     if constexpr (DMA) {
-        constexpr int OPB = BM * RB, NIW = OPB / 4096;
+        constexpr int OPB = BM * RB, NIW = OPB / 4096, NIWB = BN * RB / 4096;  // A bytes; wave loads of A, B
         // this lane's DMA units: tile row and the first k (even) of the logical vector its LDS slot holds
-        int lrow[NIW], lk[NIW];
+        // (units 0 .. NIW-1 of A; the B units are the same for u < NIW and continue the rows past it)
+        int lrow[NIWB], lk[NIWB];
+#pragma unroll
+        for (int u = 0; u < NIWB; u++) {
+            const int qq = (wave * NIWB + u) * 64 + lane, r = qq >> 3;
+            lrow[u] = r;
+            lk[u] = 2 * ((qq & 7) ^ kswz(r));
+        }
+        int larow[NIW], lak[NIW];
 #pragma unroll
         for (int u = 0; u < NIW; u++) {
             const int qq = (wave * NIW + u) * 64 + lane, r = qq >> 3;
-            lrow[u] = r;
-            lk[u] = 2 * ((qq & 7) ^ kswz(r));
+            larow[u] = r;
+            lak[u] = 2 * ((qq & 7) ^ kswz(r));
         }
         auto chunk_pos = [&](int ch, const GemmSeg *&sg, int &k1, int &k0) {
             const bool s0 = ch < nch0;
@@ -887,12 +918,17 @@ __global__ __launch_bounds__(256) void gemm_kc_kernel(GemmArgs g) {
             char *base = lds[st];
 #pragma unroll
             for (int u = 0; u < NIW; u++) {
-                const int k = k0 + lk[u], kk = k + 1 < sg->K2 ? k : 0;  // inside the row, else its first vector
-                const int gm = min(m0 + lrow[u], g.M - 1), gn = min(n0 + lrow[u], g.N - 1);
+                const int k = k0 + lak[u], kk = k + 1 < sg->K2 ? k : 0;  // inside the row, else its first vector
+                const int gm = min(m0 + larow[u], g.M - 1);
                 __builtin_amdgcn_global_load_lds((const void *)(pa + (int64_t)gm * sg->A.sr + kk),
                                                  (lds_void *)(base + (wave * NIW + u) * 1024), 16, 0, 0);
+            }
+#pragma unroll
+            for (int u = 0; u < NIWB; u++) {
+                const int k = k0 + lk[u], kk = k + 1 < sg->K2 ? k : 0;
+                const int gn = min(n0 + lrow[u], g.N - 1);
                 __builtin_amdgcn_global_load_lds((const void *)(pb + (int64_t)gn * sg->B.sr + kk),
-                                                 (lds_void *)(base + OPB + (wave * NIW + u) * 1024), 16, 0, 0);
+                                                 (lds_void *)(base + OPB + (wave * NIWB + u) * 1024), 16, 0, 0);
             }
         };
         // after the chunk has landed: the vectors reaching past K2 get (x[k], 0) or (0, 0)
@@ -907,15 +943,21 @@ __global__ __launch_bounds__(256) void gemm_kc_kernel(GemmArgs g) {
             char *base = lds[st];
 #pragma unroll
             for (int u = 0; u < NIW; u++) {
+                const int k = k0 + lak[u];
+                if (k + 1 < sg->K2)
+                    continue;
+                const int gm = min(m0 + larow[u], g.M - 1);
+                *(d2v *)(base + (wave * NIW + u) * 1024 + lane * 16) =
+                    d2v{k < sg->K2 ? pa[(int64_t)gm * sg->A.sr + k] : 0.0, 0.0};
+            }
+#pragma unroll
+            for (int u = 0; u < NIWB; u++) {
                 const int k = k0 + lk[u];
                 if (k + 1 < sg->K2)
                     continue;
-                const int gm = min(m0 + lrow[u], g.M - 1), gn = min(n0 + lrow[u], g.N - 1);
-                const bool one = k < sg->K2;
-                *(d2v *)(base + (wave * NIW + u) * 1024 + lane * 16) =
-                    d2v{one ? pa[(int64_t)gm * sg->A.sr + k] : 0.0, 0.0};
-                *(d2v *)(base + OPB + (wave * NIW + u) * 1024 + lane * 16) =
-                    d2v{one ? pb[(int64_t)gn * sg->B.sr + k] : 0.0, 0.0};
+                const int gn = min(n0 + lrow[u], g.N - 1);
+                *(d2v *)(base + OPB + (wave * NIWB + u) * 1024 + lane * 16) =
+                    d2v{k < sg->K2 ? pb[(int64_t)gn * sg->B.sr + k] : 0.0, 0.0};
             }
             return true;  // wave-uniform: every lane of the block sees the same chunk
         };
@@ -947,8 +989,8 @@ __global__ __launch_bounds__(256) void gemm_kc_kernel(GemmArgs g) {
 #pragma unroll
     for (int x = 0; x < BLK; x++)
 #pragma unroll
-        for (int y = 0; y < BLK; y++) {
-            const int n = n0 + wn * BLK * 16 + y * 16 + lr;
+        for (int y = 0; y < BLKN; y++) {
+            const int n = n0 + wn * BLKN * 16 + y * 16 + lr;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 // output layout of the 16x16 MFMAs: row lq + 4 r (FP64), 4 lq + r (FP32 accumulators)
@@ -1048,10 +1090,22 @@ hipError_t launch_gemm(const GemmArgs &g, int batch, hipStream_t s) {
             dma = dma && g.seg[q].K2 >= 2 && g.M >= 1 && g.N >= 1;
         if (const char *e = std::getenv("SCESS_GEMM_DMA"))
             dma = dma && std::atoi(e) != 0;
+        // 64 x 128 tiles for the direct-to-LDS FP64 and pre-split BF16x3 products: twice the MFMAs per
+        // barrier for 1.5x the staged bytes (CCSD update at o=50, v=500: FP64 17.9 s against 19.5 s,
+        // BF16x3 6.66 s against 7.47 s with 64 x 64). SCESS_GEMM_WIDE=0 keeps 64 x 64 (tuning).
+        const char *ew = std::getenv("SCESS_GEMM_WIDE");
+        const bool wide = dma && blk == 2 && (g.prec == P64 || g.prec == PB3S) && !(ew && std::atoi(ew) == 0);
 #define SCESS_KC(B, V, P, D)                                                                      \
     if (blk == B && (kc == 2) == V && g.prec == P && dma == D) {                                  \
-        hipLaunchKernelGGL((gemm_kc_kernel<B, V, P, D>), grid, dim3(256), 0, s, g);               \
+        hipLaunchKernelGGL((gemm_kc_kernel<B, V, P, D, 1>), grid, dim3(256), 0, s, g);            \
     } else
+        if (wide) {  // 64 x 128 tiles
+            const dim3 gw((unsigned)(((g.M + 63) / 64) * ((g.N + 127) / 128)), (unsigned)batch, (unsigned)g.splits);
+            if (g.prec == PB3S)
+                hipLaunchKernelGGL((gemm_kc_kernel<2, true, PB3S, true, 2>), gw, dim3(256), 0, s, g);
+            else
+                hipLaunchKernelGGL((gemm_kc_kernel<2, true, P64, true, 2>), gw, dim3(256), 0, s, g);
+        } else
         SCESS_KC(2, true, P64, true) SCESS_KC(2, true, P32, true) SCESS_KC(2, true, PB3, true)
         SCESS_KC(2, true, PB3S, true) SCESS_KC(2, true, PB3S, false)
         SCESS_KC(2, true, P64, false) SCESS_KC(2, false, P64, false) SCESS_KC(2, true, P32, false)
