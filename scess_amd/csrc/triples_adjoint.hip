@@ -425,7 +425,7 @@ constexpr int KN_TILE = 16 * 64;  // doubles per operand chunk (8 KiB)
 
 // source of the 16-byte vector that lane `lane` of wave load L (0..7) puts into an operand chunk;
 // mfold > 0 (A of a folded batch): tile row x is row x % mfold of batch entry x / mfold
-template <int MK>
+template <int MK, int W = 64>
 __device__ inline const double *kn_src(const GemmOp &op, int64_t base, int x0, int xext, int k1, int k2c, int K2, int L,
                                        int lane, int mfold) {
     if constexpr (MK) {  // [m][k] tile: wave load L covers rows 8 L .. 8 L + 7
@@ -439,8 +439,8 @@ __device__ inline const double *kn_src(const GemmOp &op, int64_t base, int x0, i
             m -= b * mfold;
         }
         return op.base + base + (int64_t)m * op.sr + (int64_t)k1 * op.s1 + k2;
-    } else {  // [k][x] tile: wave load L covers rows 2 L, 2 L + 1
-        const int kk = 2 * L + (lane >> 5), p = lane & 31;
+    } else {  // [k][x] tile of W-double rows: wave load L covers rows 2 L, 2 L + 1 (W = 64) or row L (128)
+        const int kk = W == 64 ? 2 * L + (lane >> 5) : L, p = W == 64 ? lane & 31 : lane;
         const int lv = p ^ ((kk & 1) << 3);
         int64_t x;
         if (mfold) {  // mfold is even, so the two rows of a vector lie in one batch entry
@@ -455,18 +455,23 @@ __device__ inline const double *kn_src(const GemmOp &op, int64_t base, int x0, i
     }
 }
 
-// element (x, k) of an operand chunk in LDS (x = row of A or B within the tile, k = 0..15)
-template <int MK> __device__ inline double kn_frag(const double *t, int x, int k) {
+// element (x, k) of an operand chunk in LDS (x = row of A or B within the tile, k = 0..15; [k][x] tiles
+// with rows of W doubles)
+template <int MK, int W = 64> __device__ inline double kn_frag(const double *t, int x, int k) {
     if constexpr (MK)
         return t[x * 16 + (((k >> 1) ^ ((x >> 1) & 7)) << 1) + (k & 1)];
     else
-        return t[k * 64 + (((x >> 1) ^ ((k & 1) << 3)) << 1) + (x & 1)];
+        return t[k * W + (((x >> 1) ^ ((k & 1) << 3)) << 1) + (x & 1)];
 }
 
-template <int AM0, int AM1, int NST>
+// WIDE = 2: tiles of 64 rows by 128 columns (each wave 32 x 64, four B wave loads per chunk), twice the
+// MFMAs per barrier for 1.5x the staged bytes; two stages only.
+template <int AM0, int AM1, int NST, int WIDE>
 __global__ __launch_bounds__(256) void gemm_kn_kernel(GemmGroup gg) {
     static_assert(NST == 2 || NST == 3, "stages");
-    __shared__ __attribute__((aligned(1024))) double lds[NST][2 * KN_TILE];  // [stage][A chunk, B chunk]
+    static_assert(WIDE == 1 || (WIDE == 2 && NST == 2), "wide tiles: two stages");
+    constexpr int BW = 64 * WIDE, NBY = 2 * WIDE;  // tile columns; MFMA blocks per wave along n
+    __shared__ __attribute__((aligned(1024))) double lds[NST][KN_TILE + 16 * BW];  // [stage][A chunk, B chunk]
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = wave >> 1, wn = wave & 1;
@@ -476,29 +481,31 @@ __global__ __launch_bounds__(256) void gemm_kn_kernel(GemmGroup gg) {
     const int beta = byz % ny, zz = byz / ny;
     const int prob = zz % gg.n, split = zz / gg.n;
     const GemmArgs &g = gg.g[prob];
-    const int ntn = (g.N + GT - 1) / GT;
+    const int ntn = (g.N + BW - 1) / BW;
     const int tm = bx / ntn, tn = bx - tm * ntn;
-    const int m0 = tm * GT, n0 = tn * GT;
+    const int m0 = tm * GT, n0 = tn * BW;
 
     const int nk20 = (g.seg[0].K2 + 15) / 16, nk21 = g.nseg > 1 ? (g.seg[1].K2 + 15) / 16 : 1;
     const int nch0 = g.seg[0].K1 * nk20, nch1 = g.nseg > 1 ? g.seg[1].K1 * nk21 : 0;
     const int total = nch0 + nch1;
     const int c_begin = (int)((int64_t)total * split / g.splits), c_end = (int)((int64_t)total * (split + 1) / g.splits);
 
-    d4 acc[2][2];
+    d4 acc[2][NBY];
 #pragma unroll
     for (int x = 0; x < 2; x++)
 #pragma unroll
-        for (int y = 0; y < 2; y++)
+        for (int y = 0; y < NBY; y++)
             acc[x][y] = d4{0.0, 0.0, 0.0, 0.0};
-    bool vm[2], vn[2];
+    bool vm[2], vn[NBY];
 #pragma unroll
-    for (int x = 0; x < 2; x++) {
+    for (int x = 0; x < 2; x++)
         vm[x] = m0 + wm * 32 + x * 16 < g.M;
-        vn[x] = n0 + wn * 32 + x * 16 < g.N;
-    }
+#pragma unroll
+    for (int y = 0; y < NBY; y++)
+        vn[y] = n0 + wn * 32 * WIDE + y * 16 < g.N;
 
-    // wave loads 2 wave and 2 wave + 1 of each operand chunk are this wave's
+    // wave loads 2 wave and 2 wave + 1 of the A chunk and 2 WIDE wave .. 2 WIDE wave + 2 WIDE - 1 of the
+    // B chunk are this wave's
     auto issue_seg = [&](const GemmSeg &sg, auto amk, int cl, int nk2, int st) {
         constexpr int MK = decltype(amk)::value;
         const int k1 = cl / nk2, k2c = cl - k1 * nk2;
@@ -507,8 +514,12 @@ __global__ __launch_bounds__(256) void gemm_kn_kernel(GemmGroup gg) {
         for (int h = 0; h < 2; h++) {
             const int L = 2 * wave + h;
             const double *sa = kn_src<MK>(sg.A, (int64_t)beta * sg.A.sb, m0, g.M, k1, k2c, sg.K2, L, lane, g.mfold);
-            const double *sb = kn_src<0>(sg.B, (int64_t)beta * sg.B.sb, n0, g.N, k1, k2c, sg.K2, L, lane, 0);
             __builtin_amdgcn_global_load_lds((const void *)sa, (lds_void *)(As + L * 128), 16, 0, 0);
+        }
+#pragma unroll
+        for (int h = 0; h < 2 * WIDE; h++) {
+            const int L = 2 * WIDE * wave + h;
+            const double *sb = kn_src<0, BW>(sg.B, (int64_t)beta * sg.B.sb, n0, g.N, k1, k2c, sg.K2, L, lane, 0);
             __builtin_amdgcn_global_load_lds((const void *)sb, (lds_void *)(Bs + L * 128), 16, 0, 0);
         }
     };
@@ -524,18 +535,18 @@ __global__ __launch_bounds__(256) void gemm_kn_kernel(GemmGroup gg) {
 #pragma unroll
         for (int t = 0; t < 4; t++) {
             const int kk = t * 4 + (lane >> 4);
-            double af[2], bf[2];
+            double af[2], bf[NBY];
 #pragma unroll
             for (int x = 0; x < 2; x++)
                 af[x] = kn_frag<MK>(As, wm * 32 + x * 16 + (lane & 15), kk);
 #pragma unroll
-            for (int y = 0; y < 2; y++)
-                bf[y] = kn_frag<0>(Bs, wn * 32 + y * 16 + (lane & 15), kk);
+            for (int y = 0; y < NBY; y++)
+                bf[y] = kn_frag<0, BW>(Bs, wn * 32 * WIDE + y * 16 + (lane & 15), kk);
 #pragma unroll
             for (int x = 0; x < 2; x++)
                 if (vm[x])
 #pragma unroll
-                    for (int y = 0; y < 2; y++)
+                    for (int y = 0; y < NBY; y++)
                         if (vn[y])
                             acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[x], bf[y], acc[x][y], 0, 0, 0);
         }
@@ -564,8 +575,8 @@ __global__ __launch_bounds__(256) void gemm_kn_kernel(GemmGroup gg) {
 #pragma unroll
     for (int x = 0; x < 2; x++)
 #pragma unroll
-        for (int y = 0; y < 2; y++) {
-            const int n = n0 + wn * 32 + y * 16 + (lane & 15);
+        for (int y = 0; y < NBY; y++) {
+            const int n = n0 + wn * 32 * WIDE + y * 16 + (lane & 15);
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int m = m0 + wm * 32 + x * 16 + (lane >> 4) + 4 * r;
@@ -1164,23 +1175,36 @@ static hipError_t launch_kn_group(const GemmGroup &gg0, int am, int batch, hipSt
         batch = 1;
     }
     const GemmArgs &g = gg.g[0];
-    const int ntiles = ((g.M + GT - 1) / GT) * ((g.N + GT - 1) / GT);
-    const dim3 grid((unsigned)ntiles, (unsigned)batch, (unsigned)(g.splits * gg.n));
     // am: 1 + (A of segment 0 unit-stride in k) + 2 (A of segment 1 unit-stride in k)
     const char *e3 = std::getenv("SCESS_KN_STAGES");  // tuning override: 2 or 3 LDS stages
-    if (e3 && std::atoi(e3) == 3) {
+    const char *ew = std::getenv("SCESS_KN_WIDE");    // tuning override: 1 for 64 x 128 tiles
+    const bool three = e3 && std::atoi(e3) == 3;
+    // 64 x 128 tiles where N spans more than one 64-wide tile: slower here (4.07 s per backward at o=20,
+    // v=200 against 3.70 s), as fewer resident work-groups cost these short-K GEMMs more than the MFMAs
+    // per barrier gain
+    const bool wide = !three && g.N > 64 && ew && std::atoi(ew) == 1;
+    const int ntiles = ((g.M + GT - 1) / GT) * ((g.N + (wide ? 2 * GT : GT) - 1) / (wide ? 2 * GT : GT));
+    const dim3 grid((unsigned)ntiles, (unsigned)batch, (unsigned)(g.splits * gg.n));
+    if (wide) {
         switch (am) {
-        case 1: hipLaunchKernelGGL((gemm_kn_kernel<0, 0, 3>), grid, dim3(256), 0, s, gg); break;
-        case 2: hipLaunchKernelGGL((gemm_kn_kernel<1, 0, 3>), grid, dim3(256), 0, s, gg); break;
-        case 3: hipLaunchKernelGGL((gemm_kn_kernel<0, 1, 3>), grid, dim3(256), 0, s, gg); break;
-        default: hipLaunchKernelGGL((gemm_kn_kernel<1, 1, 3>), grid, dim3(256), 0, s, gg); break;
+        case 1: hipLaunchKernelGGL((gemm_kn_kernel<0, 0, 2, 2>), grid, dim3(256), 0, s, gg); break;
+        case 2: hipLaunchKernelGGL((gemm_kn_kernel<1, 0, 2, 2>), grid, dim3(256), 0, s, gg); break;
+        case 3: hipLaunchKernelGGL((gemm_kn_kernel<0, 1, 2, 2>), grid, dim3(256), 0, s, gg); break;
+        default: hipLaunchKernelGGL((gemm_kn_kernel<1, 1, 2, 2>), grid, dim3(256), 0, s, gg); break;
+        }
+    } else if (three) {
+        switch (am) {
+        case 1: hipLaunchKernelGGL((gemm_kn_kernel<0, 0, 3, 1>), grid, dim3(256), 0, s, gg); break;
+        case 2: hipLaunchKernelGGL((gemm_kn_kernel<1, 0, 3, 1>), grid, dim3(256), 0, s, gg); break;
+        case 3: hipLaunchKernelGGL((gemm_kn_kernel<0, 1, 3, 1>), grid, dim3(256), 0, s, gg); break;
+        default: hipLaunchKernelGGL((gemm_kn_kernel<1, 1, 3, 1>), grid, dim3(256), 0, s, gg); break;
         }
     } else {
         switch (am) {
-        case 1: hipLaunchKernelGGL((gemm_kn_kernel<0, 0, 2>), grid, dim3(256), 0, s, gg); break;
-        case 2: hipLaunchKernelGGL((gemm_kn_kernel<1, 0, 2>), grid, dim3(256), 0, s, gg); break;
-        case 3: hipLaunchKernelGGL((gemm_kn_kernel<0, 1, 2>), grid, dim3(256), 0, s, gg); break;
-        default: hipLaunchKernelGGL((gemm_kn_kernel<1, 1, 2>), grid, dim3(256), 0, s, gg); break;
+        case 1: hipLaunchKernelGGL((gemm_kn_kernel<0, 0, 2, 1>), grid, dim3(256), 0, s, gg); break;
+        case 2: hipLaunchKernelGGL((gemm_kn_kernel<1, 0, 2, 1>), grid, dim3(256), 0, s, gg); break;
+        case 3: hipLaunchKernelGGL((gemm_kn_kernel<0, 1, 2, 1>), grid, dim3(256), 0, s, gg); break;
+        default: hipLaunchKernelGGL((gemm_kn_kernel<1, 1, 2, 1>), grid, dim3(256), 0, s, gg); break;
         }
     }
     hipError_t e = hipGetLastError();
